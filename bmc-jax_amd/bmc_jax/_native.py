@@ -1,0 +1,169 @@
+"""ctypes binding of libbmc_hip.so (include/bmc_hip.h) and the device-memory plumbing around it.
+
+PyTorch-ROCm is used here only for device memory and the current stream; every number is computed by
+the library's HIP kernels. There is no CPU fallback: without the library or a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libbmc_hip.so")
+
+BMC_F32, BMC_F64 = 0, 1
+LM_EARLY_EXIT, LM_GRAPH = 1, 2
+MAX_CEST = 4
+MAX_TANGENTS = 16
+
+# packed record layout (include/bmc_hip.h)
+SCALAR_SLOTS = {"water_r1": 0, "water_r2": 1, "water_f": 2, "mt_k": 3, "mt_f": 4, "mt_dw": 5, "mt_r1": 6,
+                "mt_r2": 7, "mt_lineshape": 8, "b0": 9, "gamma": 10, "b0_inhom": 11, "rel_b1": 12}
+POOL_FIELDS = ("cest_k", "cest_f", "cest_dw", "cest_r1", "cest_r2")
+
+_lib = None
+_lock = threading.Lock()
+
+
+class NativeError(RuntimeError):
+    """A libbmc_hip call failed (message from bmc_last_error)."""
+
+
+def slot_index(field: str, index, n_cest: int) -> int:
+    """Packed-record position of a SimParams field (and pool index for cest_* fields)."""
+    if field in SCALAR_SLOTS:
+        return SCALAR_SLOTS[field]
+    if field in POOL_FIELDS:
+        if index is None or not 0 <= int(index) < n_cest:
+            raise ValueError(f"{field}: pool index {index} outside 0..{n_cest - 1}")
+        return 13 + POOL_FIELDS.index(field) * n_cest + int(index)
+    raise ValueError(f"{field!r} is not a SimParams field")
+
+
+def lib():
+    """The loaded library. Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(f"libbmc_hip.so is not built ({LIB_PATH}); run python bmc-jax_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip32, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.bmc_abi_version.restype = ctypes.c_int
+        L.bmc_last_error.restype = ctypes.c_char_p
+        L.bmc_device_count.restype = ctypes.c_int
+        L.bmc_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, ip32, dp, dp, dp, dp, ctypes.c_int,
+                                      ctypes.POINTER(vp)]
+        L.bmc_plan_destroy.argtypes = [vp]
+        L.bmc_plan_stats.argtypes = [vp, i64p, i64p, i64p]
+        L.bmc_simulate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32, dp,
+                                   ctypes.c_int, vp, vp, vp]
+        L.bmc_lm_fit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ip32, dp, dp, dp,
+                                 ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_uint,
+                                 ctypes.c_double, ctypes.c_double, vp, vp, vp, vp]
+        for name in ("bmc_plan_create", "bmc_plan_destroy", "bmc_plan_stats", "bmc_simulate", "bmc_lm_fit"):
+            getattr(L, name).restype = ctypes.c_int
+        if L.bmc_abi_version() != 1:
+            raise NativeError("libbmc_hip.so ABI version mismatch")
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().bmc_last_error().decode(errors="replace")
+        raise NativeError(f"{what} failed ({rc}): {msg}")
+
+
+def torch_cuda():
+    """torch with a visible GPU, or a loud error: the product path has no CPU fallback."""
+    import torch
+    if not torch.cuda.is_available():
+        raise NativeError("no GPU visible: the BMC kernels run on HIP devices only")
+    return torch
+
+
+def stream_ptr():
+    torch = torch_cuda()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Plan:
+    """A sequence compiled for the device (bmc_plan_create): events [O, T] and the row programs."""
+
+    def __init__(self, type_, dt, rf_amp, rf_phase, rf_freq, reset_after_adc=True):
+        L = lib()
+        torch_cuda()
+        t = np.ascontiguousarray(np.asarray(type_), dtype=np.int32)
+        if t.ndim != 2:
+            raise ValueError(f"events must be (n_offsets, n_events), got shape {t.shape}")
+        f = lambda a: np.ascontiguousarray(np.asarray(a, np.float64))
+        self._arrays = (t, f(dt), f(rf_amp), f(rf_phase), f(rf_freq))
+        for a in self._arrays[1:]:
+            if a.shape != t.shape:
+                raise ValueError("event fields disagree in shape")
+        self.n_offsets, self.n_events = t.shape
+        h = ctypes.c_void_p()
+        check(L.bmc_plan_create(self.n_offsets, self.n_events, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                *[_dp(a) for a in self._arrays[1:]], 1 if reset_after_adc else 0, ctypes.byref(h)),
+              "bmc_plan_create")
+        self._h = h
+        self._arrays = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().bmc_plan_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "bmc_plan_stats")
+        return dict(n_ops=a.value, n_built=b.value, n_applied=c.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            try:
+                _lib.bmc_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def device_array(x, dtype):
+    """numpy -> contiguous device tensor of the given torch dtype."""
+    torch = torch_cuda()
+    t = torch.as_tensor(np.ascontiguousarray(x))
+    return t.to(device="cuda", dtype=dtype).contiguous()
+
+
+def simulate(plan: Plan, params, n_cest: int, dtype_code: int, m0, dm0=None, slots=(), scales=(), jc=0,
+             want_m=True, want_jac=True):
+    """bmc_simulate on device tensors. params [V, S], m0 [V, N] torch tensors of the working dtype.
+
+    Returns (m [V, O, N] or None, jac [V, O, P] or None) as device tensors.
+    """
+    torch = torch_cuda()
+    L = lib()
+    V = int(params.shape[0])
+    N = 3 * (1 + n_cest) + 1
+    P = len(slots)
+    tdt = torch.float64 if dtype_code == BMC_F64 else torch.float32
+    m_out = torch.empty((V, plan.n_offsets, N), dtype=tdt, device="cuda") if want_m else None
+    jac = torch.empty((V, plan.n_offsets, P), dtype=tdt, device="cuda") if (want_jac and P) else None
+    sl = (ctypes.c_int32 * max(P, 1))(*[int(s) for s in slots])
+    sc = (ctypes.c_double * max(P, 1))(*[float(s) for s in scales])
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    check(L.bmc_simulate(plan.handle, V, n_cest, dtype_code, ptr(params), ptr(m0), ptr(dm0), P, sl, sc, jc,
+                         ptr(m_out), ptr(jac), stream_ptr()), "bmc_simulate")
+    return m_out, jac

@@ -1,0 +1,268 @@
+// C ABI of libbmc_hip.so: sequence plans (host-compiled row programs) and the simulate entry point.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bmc_internal.h"
+
+namespace bmc {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+
+struct EvKey {
+  int32_t type;
+  double dt, amp, ph, fr;
+  bool operator==(const EvKey& o) const {
+    return type == o.type && memcmp(&dt, &o.dt, 8) == 0 && memcmp(&amp, &o.amp, 8) == 0 &&
+           memcmp(&ph, &o.ph, 8) == 0 && memcmp(&fr, &o.fr, 8) == 0;
+  }
+};
+
+struct EvKeyHash {
+  size_t operator()(const EvKey& k) const {
+    uint64_t h = 1469598103934665603ull ^ (uint64_t)k.type;
+    const double* d[4] = {&k.dt, &k.amp, &k.ph, &k.fr};
+    for (int i = 0; i < 4; i++) {
+      uint64_t b;
+      memcpy(&b, d[i], 8);
+      h = (h ^ b) * 1099511628211ull;
+    }
+    return (size_t)h;
+  }
+};
+
+// Greedy run detection for one row: a block [i, i + p) repeated c >= 2 times back to back, with no
+// ADC inside, becomes one BLOCK op; everything else is SEQ. Candidate periods come from the next few
+// occurrences of event i's exact values.
+void compile_row(const std::vector<EvKey>& row, std::vector<Op>& ops, int64_t& built, int64_t& applied) {
+  const int T = (int)row.size();
+  std::unordered_map<EvKey, std::vector<int>, EvKeyHash> pos;
+  for (int e = 0; e < T; e++) pos[row[e]].push_back(e);
+  std::vector<int> next_adc(T + 1, T);
+  for (int e = T - 1; e >= 0; e--) next_adc[e] = (row[e].type == BMC_EV_ADC) ? e : next_adc[e + 1];
+  auto flush_seq = [&](int s0, int s1) {
+    if (s1 > s0) {
+      ops.push_back(Op{OP_SEQ, s0, s1 - s0, 1});
+      built += s1 - s0;
+      applied += s1 - s0;
+    }
+  };
+  int seq_start = 0, i = 0;
+  while (i < T) {
+    int best_p = 0, best_c = 0;
+    const std::vector<int>& occ = pos[row[i]];
+    auto it = std::upper_bound(occ.begin(), occ.end(), i);
+    for (int tries = 0; it != occ.end() && tries < 8; ++it, ++tries) {
+      const int p = *it - i;
+      if (next_adc[i] < i + p) break;  // the block would hold an ADC
+      int c = 1;
+      while (i + (c + 1) * p <= T) {
+        bool same = true;
+        for (int t = 0; t < p && same; t++) same = row[i + c * p + t] == row[i + t];
+        if (!same) break;
+        c++;
+      }
+      if (c >= 2 && (long)c * p > (long)best_c * best_p) {
+        best_p = p;
+        best_c = c;
+      }
+    }
+    if (best_c >= 2) {
+      flush_seq(seq_start, i);
+      ops.push_back(Op{OP_BLOCK, i, best_p, best_c});
+      built += best_p;
+      applied += best_c;
+      i += best_p * best_c;
+      seq_start = i;
+    } else {
+      i++;
+    }
+  }
+  flush_seq(seq_start, T);
+}
+
+}  // namespace
+}  // namespace bmc
+
+using namespace bmc;
+
+extern "C" {
+
+int bmc_abi_version(void) { return BMC_ABI_VERSION; }
+
+const char* bmc_last_error(void) { return g_err.c_str(); }
+
+int bmc_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
+                    const double* rf_phase, const double* rf_freq, int reset_after_adc, bmc_plan** out) {
+  if (!out || n_offsets <= 0 || n_events < 0 || (n_events > 0 && (!type || !dt || !rf_amp || !rf_phase || !rf_freq))) {
+    set_error("bmc_plan_create: bad arguments");
+    return BMC_EINVAL;
+  }
+  *out = nullptr;
+  const int O = n_offsets, T = n_events;
+  std::vector<Ev> ev((size_t)O * T);
+  std::vector<Op> ops;
+  std::vector<int32_t> row_ptr(O + 1, 0);
+  int64_t built = 0, applied = 0;
+  for (int o = 0; o < O; o++) {
+    std::vector<EvKey> row(T);
+    for (int e = 0; e < T; e++) {
+      const size_t k = (size_t)o * T + e;
+      if (type[k] < 0 || type[k] > 4 || !(dt[k] >= 0.0) || !isfinite(dt[k])) {
+        set_error("bmc_plan_create: event type must be 0..4 and dt finite and >= 0");
+        return BMC_EINVAL;
+      }
+      row[e] = EvKey{type[k], dt[k], rf_amp[k], rf_phase[k], rf_freq[k]};
+      ev[k] = Ev{dt[k], rf_amp[k], cos(rf_phase[k]), sin(rf_phase[k]), rf_freq[k], type[k], 0};
+    }
+    row_ptr[o] = (int32_t)ops.size();
+    compile_row(row, ops, built, applied);
+  }
+  row_ptr[O] = (int32_t)ops.size();
+  bmc_plan* pl = new bmc_plan();
+  Plan& p = pl->p;
+  p.O = O;
+  p.T = T;
+  p.reset_after_adc = reset_after_adc ? 1 : 0;
+  p.n_ops = (int64_t)ops.size();
+  p.n_built = built;
+  p.n_applied = applied;
+  auto fail = [&](hipError_t e) {
+    set_error(std::string("bmc_plan_create: ") + hipGetErrorString(e));
+    bmc_plan_destroy(pl);
+    return e == hipErrorOutOfMemory ? BMC_ENOMEM : BMC_EHIP;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&p.d_ev, sizeof(Ev) * std::max<size_t>(ev.size(), 1))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&p.d_ops, sizeof(Op) * std::max<size_t>(ops.size(), 1))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&p.d_row_ptr, sizeof(int32_t) * row_ptr.size())) != hipSuccess) return fail(e);
+  if (!ev.empty() && (e = hipMemcpy(p.d_ev, ev.data(), sizeof(Ev) * ev.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e);
+  if (!ops.empty() && (e = hipMemcpy(p.d_ops, ops.data(), sizeof(Op) * ops.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e);
+  if ((e = hipMemcpy(p.d_row_ptr, row_ptr.data(), sizeof(int32_t) * row_ptr.size(), hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return fail(e);
+  *out = pl;
+  return BMC_OK;
+}
+
+int bmc_plan_destroy(bmc_plan* plan) {
+  if (!plan) return BMC_OK;
+  Plan& p = plan->p;
+  if (p.d_ev) (void)hipFree(p.d_ev);
+  if (p.d_ops) (void)hipFree(p.d_ops);
+  if (p.d_row_ptr) (void)hipFree(p.d_row_ptr);
+  delete plan;
+  return BMC_OK;
+}
+
+int bmc_plan_stats(const bmc_plan* plan, int64_t* n_ops, int64_t* n_built, int64_t* n_applied) {
+  if (!plan) {
+    set_error("bmc_plan_stats: null plan");
+    return BMC_EINVAL;
+  }
+  if (n_ops) *n_ops = plan->p.n_ops;
+  if (n_built) *n_built = plan->p.n_built;
+  if (n_applied) *n_applied = plan->p.n_applied;
+  return BMC_OK;
+}
+
+}  // extern "C"
+
+namespace bmc {
+
+// Fills the parts of SimArgs common to simulate and the LM loop. Returns BMC_OK or an error.
+int fill_sim_args(const bmc_plan* plan, int V, int nc, int dtype, const void* params, const void* m0,
+                  const void* dm0, int P, const int32_t* slot, const double* scale, int jc, SimArgs& a) {
+  if (!plan) {
+    set_error("null plan");
+    return BMC_EINVAL;
+  }
+  if (V < 0 || nc < 0 || (dtype != BMC_F32 && dtype != BMC_F64) || !params || !m0) {
+    set_error("bad sizes, dtype or null input");
+    return BMC_EINVAL;
+  }
+  if (nc > BMC_MAX_CEST) {
+    set_error("n_cest " + std::to_string(nc) + " exceeds the compiled maximum " + std::to_string(BMC_MAX_CEST));
+    return BMC_ENOTSUP;
+  }
+  const int N = 3 * (1 + nc) + 1;
+  if (P < 0 || P > BMC_MAX_TANGENTS || (P > 0 && (!slot || !scale))) {
+    set_error("tangent count must be 0.." + std::to_string(BMC_MAX_TANGENTS));
+    return BMC_EINVAL;
+  }
+  if (jc < 0 || jc >= N) {
+    set_error("jac_component out of range");
+    return BMC_EINVAL;
+  }
+  memset(&a, 0, sizeof(a));
+  const Plan& p = plan->p;
+  a.ev = p.d_ev;
+  a.ops = p.d_ops;
+  a.row_ptr = p.d_row_ptr;
+  a.O = p.O;
+  a.T = p.T;
+  a.reset_after_adc = p.reset_after_adc;
+  a.V = V;
+  a.dtype = dtype;
+  a.stride = 13 + 5 * nc;
+  a.params = params;
+  a.m0 = m0;
+  a.dm0 = dm0;
+  a.P = P;
+  for (int i = 0; i < P; i++) {
+    int idx = 0;
+    int k = tan_kind_of_slot(slot[i], nc, &idx);
+    if (k < 0) {
+      set_error("tangent slot " + std::to_string(slot[i]) + " outside the packed record");
+      return BMC_EINVAL;
+    }
+    a.t_kind[i] = k;
+    a.t_idx[i] = idx;
+    a.t_scale[i] = scale[i];
+  }
+  a.jc = jc;
+  // reference NaN rule (expm.py:73-83): floor(log2(l1 / theta)) > 16  <=>  l1 >= theta 2^17
+  a.nan_l1 = (dtype == BMC_F32 ? 3.925724783138660 : 5.371920351148152) * 131072.0;
+  return BMC_OK;
+}
+
+// Tangents per pass for a tangent count (register budget of the kernel).
+int tangents_per_pass(int P) { return P <= 0 ? 0 : 1; }
+
+}  // namespace bmc
+
+extern "C" int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                            const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
+                            const double* tangent_scale, int jac_component, void* m_out_dev, void* jac_out_dev,
+                            void* stream) {
+  SimArgs a;
+  int rc = fill_sim_args(plan, n_voxels, n_cest, dtype, params_dev, m0_dev, dm0_dev, n_tangents, tangent_slot,
+                         tangent_scale, jac_component, a);
+  if (rc != BMC_OK) return rc;
+  if (n_voxels == 0) return BMC_OK;
+  a.m_out = m_out_dev;
+  a.jac_out = jac_out_dev;
+  a.jac_dtype = dtype;
+  const int want_tan = (jac_out_dev && n_tangents > 0) ? n_tangents : 0;
+  a.PT = tangents_per_pass(want_tan);
+  a.n_pass = want_tan ? (want_tan + a.PT - 1) / a.PT : 1;
+  if (!want_tan) a.P = n_tangents;  // seeds unused
+  return launch_sim(n_cest, a, (hipStream_t)stream);
+}

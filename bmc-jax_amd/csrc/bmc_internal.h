@@ -1,0 +1,106 @@
+// Internal declarations shared by the HIP translation units of libbmc_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "bmc_hip.h"
+
+namespace bmc {
+
+// One event, widened to float64 on upload. cos/sin of the phase are taken on the host once.
+struct Ev {
+  double dt, amp, cph, sph, fr;
+  int32_t type, pad;
+};
+
+// One instruction of a row program.
+//   kind 0 (SEQ):   events [e0, e0 + len) applied one by one to the state.
+//   kind 1 (BLOCK): the propagator of events [e0, e0 + len) is chained once, then applied `count`
+//                   times; the block holds no ADC.
+struct Op {
+  int32_t kind, e0, len, count;
+};
+
+enum { OP_SEQ = 0, OP_BLOCK = 1 };
+
+// Tangent kinds: how the generator depends on a packed slot (matrix.py:83-295 differentiated).
+enum TanKind {
+  TK_NONE = 0,
+  TK_W_R1,
+  TK_W_R2,
+  TK_MT_K,
+  TK_MT_F,
+  TK_MT_R1,
+  TK_MT_DW,
+  TK_MT_R2,
+  TK_B0,
+  TK_GAMMA,
+  TK_B0INH,
+  TK_RELB1,
+  TK_C_K,
+  TK_C_F,
+  TK_C_DW,
+  TK_C_R1,
+  TK_C_R2
+};
+
+constexpr int kMaxTan = BMC_MAX_TANGENTS;
+
+struct SimArgs {
+  const Ev* ev;           // [O, T]
+  const Op* ops;          // program
+  const int32_t* row_ptr; // [O + 1]
+  int O, T, reset_after_adc;
+  int V, dtype, stride;
+  const void* params;     // [V, stride] dtype
+  const void* m0;         // [V, N] dtype
+  const void* dm0;        // [V, P, N] dtype or null
+  // values written into slots before simulating (fit_batch's write_params): [V, n_write] float64
+  const double* xw;
+  int n_write;
+  int w_slot[kMaxTan];
+  double w_scale[kMaxTan];
+  // tangents
+  int P, PT, n_pass;
+  int t_kind[kMaxTan], t_idx[kMaxTan];
+  double t_scale[kMaxTan];
+  int jc;
+  void* m_out;            // [V, O, N] dtype or null
+  double* z_out;          // [V, O] float64 or null: state[jc]
+  void* jac_out;          // [V, O, P] (jac_dtype) or null
+  int jac_dtype;
+  const int32_t* active;  // [V] or null: skip voxels whose flag is 0
+  double nan_l1;          // reference NaN threshold on the augmented 1-norm
+};
+
+struct Plan {
+  int O = 0, T = 0, reset_after_adc = 1;
+  Ev* d_ev = nullptr;
+  Op* d_ops = nullptr;
+  int32_t* d_row_ptr = nullptr;
+  int64_t n_ops = 0, n_built = 0, n_applied = 0;
+  int max_events_per_row = 0;
+};
+
+void set_error(const std::string& msg);
+int tan_kind_of_slot(int slot, int nc, int* idx);
+int launch_sim(int nc, const SimArgs& a, hipStream_t stream);
+
+}  // namespace bmc
+
+struct bmc_plan {
+  bmc::Plan p;
+};
+
+#define BMC_HIP_CHECK(expr)                                                        \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      bmc::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));          \
+      return BMC_EHIP;                                                             \
+    }                                                                              \
+  } while (0)

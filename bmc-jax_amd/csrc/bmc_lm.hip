@@ -1,0 +1,376 @@
+// Batched bounded Levenberg-Marquardt on the GPU: BMC-JAX's levenberg_marquardt
+// (src/bmc_jax/fitting/lm.py:20-67) vmapped over voxels by fit_batch (fitting/independent.py:118-128).
+//
+// One iteration is four launches on one stream, every buffer fixed, so the iteration can be captured
+// once as a hipGraph and replayed:
+//   1. sim_kernel at x with one tangent per free value  -> z [V, O], J [V, O, P]      (float64)
+//   2. normal_eq_kernel: r = z - z_meas, c = r.r, g = J^T r, H = J^T J in float64, A = H + lam (diag H
+//      + 1e-9 I) (+ diag(1 - free)), delta = A^-1 (-g) by LU with partial pivoting, x_t = clip(x + delta)
+//   3. sim_kernel at x_t, primal only                    -> z_t [V, O]
+//   4. accept_kernel: c_t = |z_t - z_meas|^2; accept iff c_t < c (lam * 0.5) else reject (lam * 3)
+// The iteration counter lives on the device so a replayed graph advances it.
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "bmc_internal.h"
+
+namespace bmc {
+
+int fill_sim_args(const bmc_plan* plan, int V, int nc, int dtype, const void* params, const void* m0,
+                  const void* dm0, int P, const int32_t* slot, const double* scale, int jc, SimArgs& a);
+
+namespace {
+
+constexpr int kMaxP = BMC_MAX_TANGENTS;
+
+struct LmState {
+  double* x;        // [V, P] current (scaled)
+  double* xt;       // [V, P] trial
+  double* lam;      // [V]
+  double* cost;     // [V] cost at x (set by normal_eq)
+  int32_t* active;  // [V]
+  int32_t* it;      // [1] iteration counter
+  int32_t* n_done;  // [V] or null
+  double* ctraj;    // [V, n_iter] or null
+  double* xtraj;    // [V, n_iter, P] or null
+};
+
+__device__ __forceinline__ double ldr(const void* p, long i, int dtype) {
+  return dtype == BMC_F64 ? static_cast<const double*>(p)[i] : (double)static_cast<const float*>(p)[i];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+struct NeArgs {
+  int V, O, P, dtype, has_free;
+  const double* z;
+  const double* J;
+  const void* zmeas;
+  const double* free_;
+  double lo[kMaxP], hi[kMaxP];
+  LmState s;
+};
+
+// One wave per voxel. Lanes accumulate J^T J entries (pairs) and J^T r over the offsets.
+__global__ void __launch_bounds__(64) normal_eq_kernel(NeArgs a) {
+  const int v = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (v >= a.V || !a.s.active[v]) return;
+  const int P = a.P, O = a.O;
+  __shared__ double H[kMaxP][kMaxP + 1];
+  __shared__ double g[kMaxP];
+  __shared__ double fr[kMaxP];
+  const double* Jv = a.J + (long)v * O * P;
+  const double* zv = a.z + (long)v * O;
+  if (lane < P) fr[lane] = a.has_free ? a.free_[(long)v * P + lane] : 1.0;
+  __syncthreads();
+  // cost and J^T r
+  double c = 0.0;
+  for (int o = lane; o < O; o += 64) {
+    double r = zv[o] - ldr(a.zmeas, (long)v * O + o, a.dtype);
+    c = fma(r, r, c);
+  }
+  c = wave_sum(c);
+  for (int p = 0; p < P; p++) {
+    double acc = 0.0;
+    for (int o = lane; o < O; o += 64) {
+      double r = zv[o] - ldr(a.zmeas, (long)v * O + o, a.dtype);
+      acc = fma(Jv[(long)o * P + p] * fr[p], r, acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) g[p] = acc;
+  }
+  // J^T J, one lane per (p, q) pair, q <= p
+  const int npair = P * (P + 1) / 2;
+  for (int pi = lane; pi < npair; pi += 64) {
+    int p = 0;
+    while ((p + 1) * (p + 2) / 2 <= pi) p++;
+    const int q = pi - p * (p + 1) / 2;
+    double acc = 0.0;
+    for (int o = 0; o < O; o++) acc = fma(Jv[(long)o * P + p], Jv[(long)o * P + q], acc);
+    acc *= fr[p] * fr[q];
+    H[p][q] = acc;
+    H[q][p] = acc;
+  }
+  __syncthreads();
+  if (lane != 0) return;
+  const double lam = a.s.lam[v];
+  a.s.cost[v] = c;
+  // the damped system in place (lm.py:53-55), then Gaussian elimination with partial pivoting
+  // (jnp.linalg.solve, lm.py:56)
+  for (int p = 0; p < P; p++) {
+    H[p][p] += lam * (H[p][p] + 1e-9);
+    if (a.has_free) H[p][p] += 1.0 - fr[p];
+    H[p][P] = -g[p];
+  }
+  for (int cI = 0; cI < P; cI++) {
+    int pr = cI;
+    double best = fabs(H[cI][cI]);
+    for (int r = cI + 1; r < P; r++)
+      if (fabs(H[r][cI]) > best) {
+        best = fabs(H[r][cI]);
+        pr = r;
+      }
+    if (pr != cI)
+      for (int q = 0; q <= P; q++) {
+        double t = H[cI][q];
+        H[cI][q] = H[pr][q];
+        H[pr][q] = t;
+      }
+    for (int r = cI + 1; r < P; r++) {
+      double m = H[r][cI] / H[cI][cI];
+      for (int q = cI; q <= P; q++) H[r][q] -= m * H[cI][q];
+    }
+  }
+  double* d = g;
+  for (int r = P - 1; r >= 0; r--) {
+    double s = H[r][P];
+    for (int q = r + 1; q < P; q++) s -= H[r][q] * d[q];
+    d[r] = s / H[r][r];
+  }
+  for (int p = 0; p < P; p++) {
+    double xn = a.s.x[(long)v * P + p] + d[p];
+    xn = fmin(fmax(xn, a.lo[p]), a.hi[p]);  // jnp.clip (lm.py:57)
+    a.s.xt[(long)v * P + p] = xn;
+  }
+}
+
+struct AcArgs {
+  int V, O, P, dtype, n_iter, early;
+  double ftol, rms_tol;
+  const double* zt;
+  const void* zmeas;
+  LmState s;
+};
+
+__global__ void __launch_bounds__(64) accept_kernel(AcArgs a) {
+  const int v = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (v >= a.V) return;
+  const int it = *a.s.it;
+  if (!a.s.active[v]) {
+    // a stopped problem keeps its last cost and x in the trajectory
+    if (lane == 0 && a.s.ctraj && it < a.n_iter) a.s.ctraj[(long)v * a.n_iter + it] = a.s.cost[v];
+    if (lane < a.P && a.s.xtraj && it < a.n_iter)
+      a.s.xtraj[((long)v * a.n_iter + it) * a.P + lane] = a.s.x[(long)v * a.P + lane];
+    return;
+  }
+  double c = 0.0;
+  const double* zv = a.zt + (long)v * a.O;
+  for (int o = lane; o < a.O; o += 64) {
+    double r = zv[o] - ldr(a.zmeas, (long)v * a.O + o, a.dtype);
+    c = fma(r, r, c);
+  }
+  c = wave_sum(c);
+  const double c0 = a.s.cost[v];
+  const bool better = c < c0;  // lm.py:58 (a NaN trial is never better)
+  if (better && lane < a.P) a.s.x[(long)v * a.P + lane] = a.s.xt[(long)v * a.P + lane];
+  if (lane < a.P && a.s.xtraj)
+    a.s.xtraj[((long)v * a.n_iter + it) * a.P + lane] = better ? a.s.xt[(long)v * a.P + lane] : a.s.x[(long)v * a.P + lane];
+  if (lane != 0) return;
+  const double lam = a.s.lam[v];
+  a.s.lam[v] = better ? lam * 0.5 : lam * 3.0;  // lm.py:59-60
+  const double cout = better ? c : c0;
+  if (better) a.s.cost[v] = c;
+  if (a.s.ctraj) a.s.ctraj[(long)v * a.n_iter + it] = cout;
+  if (a.s.n_done) a.s.n_done[v] = it + 1;
+  if (a.early) {
+    const bool small = sqrt(cout / (double)a.O) <= a.rms_tol;
+    const bool flat = better && (c0 - c) <= a.ftol * c0;
+    const bool stuck = a.s.lam[v] > 1e12;
+    if (small || flat || stuck) a.s.active[v] = 0;
+  }
+}
+
+__global__ void bump_iter_kernel(int32_t* it) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *it += 1;
+}
+
+__global__ void init_kernel(LmState s, int V, double lam0) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v == 0) *s.it = 0;
+  if (v < V) {
+    s.lam[v] = lam0;
+    s.active[v] = 1;
+    s.cost[v] = 0.0;
+    if (s.n_done) s.n_done[v] = 0;
+  }
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+}  // namespace bmc
+
+using namespace bmc;
+
+extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                          const void* m0_dev, int n_params, const int32_t* slot, const double* scale, const double* lo,
+                          const double* hi, int jac_component, const void* z_meas_dev, double* x_dev, int n_iter,
+                          double lam0, const double* free_dev, unsigned flags, double early_ftol, double early_rms,
+                          double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev, void* stream_) {
+  const int V = n_voxels, P = n_params;
+  if (P <= 0 || P > kMaxP || !lo || !hi || !x_dev || !z_meas_dev || n_iter < 0) {
+    set_error("bmc_lm_fit: need 1.." + std::to_string(kMaxP) + " free values, bounds, x and z_meas");
+    return BMC_EINVAL;
+  }
+  SimArgs sj;
+  int rc = fill_sim_args(plan, V, n_cest, dtype, params_dev, m0_dev, nullptr, P, slot, scale, jac_component, sj);
+  if (rc != BMC_OK) return rc;
+  if (V == 0 || n_iter == 0) return BMC_OK;
+  const int O = plan->p.O;
+  hipStream_t stream = (hipStream_t)stream_;
+  const bool use_graph = (flags & BMC_LM_GRAPH) != 0;
+  hipStream_t own = nullptr;
+  if (use_graph && stream == nullptr) {
+    BMC_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    stream = own;
+  }
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() {
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } sg{own};
+
+  DevBuf bz, bJ, bzt, bxt, blam, bcost, bact, bit;
+  auto alloc = [&](DevBuf& b, size_t n) -> int {
+    if (hipMalloc(&b.p, n > 0 ? n : 8) != hipSuccess) {
+      set_error("bmc_lm_fit: device allocation of " + std::to_string(n) + " bytes failed");
+      return BMC_ENOMEM;
+    }
+    return BMC_OK;
+  };
+  if ((rc = alloc(bz, sizeof(double) * V * O)) || (rc = alloc(bJ, sizeof(double) * (size_t)V * O * P)) ||
+      (rc = alloc(bzt, sizeof(double) * V * O)) || (rc = alloc(bxt, sizeof(double) * V * P)) ||
+      (rc = alloc(blam, sizeof(double) * V)) || (rc = alloc(bcost, sizeof(double) * V)) ||
+      (rc = alloc(bact, sizeof(int32_t) * V)) || (rc = alloc(bit, sizeof(int32_t))))
+    return rc;
+
+  LmState st;
+  st.x = x_dev;
+  st.xt = (double*)bxt.p;
+  st.lam = (double*)blam.p;
+  st.cost = (double*)bcost.p;
+  st.active = (int32_t*)bact.p;
+  st.it = (int32_t*)bit.p;
+  st.n_done = n_iter_done_dev;
+  st.ctraj = cost_traj_dev;
+  st.xtraj = x_traj_dev;
+
+  // launch 1: sim with tangents at x
+  sj.xw = x_dev;
+  sj.n_write = P;
+  for (int p = 0; p < P; p++) {
+    sj.w_slot[p] = slot[p];
+    sj.w_scale[p] = scale[p];
+  }
+  sj.z_out = (double*)bz.p;
+  sj.jac_out = bJ.p;
+  sj.jac_dtype = BMC_F64;
+  sj.active = st.active;
+  sj.PT = 1;
+  sj.n_pass = P;
+  // launch 3: primal at the trial point
+  SimArgs sp = sj;
+  sp.xw = st.xt;
+  sp.z_out = (double*)bzt.p;
+  sp.jac_out = nullptr;
+  sp.PT = 0;
+  sp.n_pass = 1;
+
+  NeArgs ne;
+  memset(&ne, 0, sizeof(ne));
+  ne.V = V;
+  ne.O = O;
+  ne.P = P;
+  ne.dtype = dtype;
+  ne.has_free = free_dev != nullptr;
+  ne.z = (const double*)bz.p;
+  ne.J = (const double*)bJ.p;
+  ne.zmeas = z_meas_dev;
+  ne.free_ = free_dev;
+  for (int p = 0; p < P; p++) {
+    ne.lo[p] = lo[p];
+    ne.hi[p] = hi[p];
+  }
+  ne.s = st;
+  AcArgs ac;
+  memset(&ac, 0, sizeof(ac));
+  ac.V = V;
+  ac.O = O;
+  ac.P = P;
+  ac.dtype = dtype;
+  ac.n_iter = n_iter;
+  ac.early = (flags & BMC_LM_EARLY_EXIT) ? 1 : 0;
+  ac.ftol = early_ftol;
+  ac.rms_tol = early_rms;
+  ac.zt = (const double*)bzt.p;
+  ac.zmeas = z_meas_dev;
+  ac.s = st;
+
+  hipLaunchKernelGGL(init_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, st, V, lam0);
+  BMC_HIP_CHECK(hipGetLastError());
+
+  auto iteration = [&]() -> int {
+    int r;
+    if ((r = launch_sim(n_cest, sj, stream)) != BMC_OK) return r;
+    hipLaunchKernelGGL(normal_eq_kernel, dim3(V), dim3(64), 0, stream, ne);
+    BMC_HIP_CHECK(hipGetLastError());
+    if ((r = launch_sim(n_cest, sp, stream)) != BMC_OK) return r;
+    hipLaunchKernelGGL(accept_kernel, dim3(V), dim3(64), 0, stream, ac);
+    BMC_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(bump_iter_kernel, dim3(1), dim3(64), 0, stream, st.it);
+    BMC_HIP_CHECK(hipGetLastError());
+    return BMC_OK;
+  };
+
+  if (!use_graph) {
+    for (int it = 0; it < n_iter; it++)
+      if ((rc = iteration()) != BMC_OK) return rc;
+  } else {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    BMC_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    rc = iteration();
+    hipError_t ce = hipStreamEndCapture(stream, &graph);
+    if (rc != BMC_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ce != hipSuccess) {
+      set_error(std::string("bmc_lm_fit: graph capture: ") + hipGetErrorString(ce));
+      return BMC_EHIP;
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      set_error(std::string("bmc_lm_fit: graph instantiate: ") + hipGetErrorString(ie));
+      return BMC_EHIP;
+    }
+    hipError_t le = hipSuccess;
+    for (int it = 0; it < n_iter && le == hipSuccess; it++) le = hipGraphLaunch(exec, stream);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (le != hipSuccess) {
+      set_error(std::string("bmc_lm_fit: graph launch: ") + hipGetErrorString(le));
+      return BMC_EHIP;
+    }
+  }
+  // buffers are freed on return: finish the work that uses them first
+  BMC_HIP_CHECK(hipStreamSynchronize(stream));
+  return BMC_OK;
+}

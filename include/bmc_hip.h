@@ -1,0 +1,136 @@
+/*
+ * bmc_hip.h -- C ABI of the MI355X simulate-and-fit engine (libbmc_hip.so).
+ *
+ * The reference (MPR-UKD/BMC-JAX) is pure Python/JAX and has no FFI; its drop-in boundary is the Python
+ * API. Each entry point below replaces the JAX computation behind one reference function (cited per
+ * entry, paths relative to the BMC-JAX tree). A thin Python layer (bmc-jax_amd/bmc_jax) binds these
+ * with ctypes and keeps the reference's Python signatures.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative BMC_E* code on failure; bmc_last_error()
+ *     then returns a message (thread-local).
+ *   - Pointers named *_dev are device pointers (hipMalloc / torch CUDA tensors); the caller owns all
+ *     buffers and the library never frees caller memory. `stream` is a hipStream_t (NULL = default).
+ *   - `dtype` is BMC_F32 or BMC_F64 and applies to every real-valued buffer of the call. The kernels
+ *     compute in float64 internally (CDNA4 runs FP64 FMA at the FP32 rate); float32 mode rounds only
+ *     inputs and outputs.
+ *
+ * Packed parameter record, one per voxel, `13 + 5 * n_cest` reals (simulation/params.py:18-49):
+ *   [0] water_r1 [1] water_r2 [2] water_f [3] mt_k [4] mt_f [5] mt_dw [6] mt_r1 [7] mt_r2
+ *   [8] mt_lineshape (0 none, 1 Lorentzian, 2 SuperLorentzian) [9] b0 [10] gamma [11] b0_inhom
+ *   [12] rel_b1, then cest_k[n_cest], cest_f[n_cest], cest_dw[n_cest], cest_r1[n_cest], cest_r2[n_cest].
+ * State vector (length N = 3 (1 + n_cest) + 1): x of water and pools, y of water and pools, z of water
+ * and pools, MT z (simulation/matrix.py:224-234). Water Mz sits at index 2 (1 + n_cest).
+ */
+#ifndef BMC_HIP_H
+#define BMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMC_ABI_VERSION 1
+
+#define BMC_F32 0
+#define BMC_F64 1
+
+#define BMC_OK 0
+#define BMC_EINVAL -1  /* bad shape, size or argument */
+#define BMC_ENOMEM -2  /* device allocation failed */
+#define BMC_EHIP -3    /* a HIP runtime call failed */
+#define BMC_ENOTSUP -4 /* configuration outside what the kernels support */
+
+/* Event types (simulation/events.py:12-18). */
+#define BMC_EV_DELAY 0
+#define BMC_EV_RF 1
+#define BMC_EV_GRADIENT 2
+#define BMC_EV_ADC 3
+#define BMC_EV_SPOILER 4
+
+/* Largest number of CEST pools the kernels are compiled for (state length <= 16, augmented <= 17). */
+#define BMC_MAX_CEST 4
+/* Largest number of free parameters (tangents) per call. */
+#define BMC_MAX_TANGENTS 16
+
+int bmc_abi_version(void);
+const char* bmc_last_error(void);
+
+/* Number of HIP devices visible (0 if none). */
+int bmc_device_count(void);
+
+/*
+ * A compiled saturation sequence: the event table (simulation/events.py:20-32) uploaded to the device
+ * together with a per-offset program that marks runs of identical events (a repeated pulse and its
+ * gap) so their propagator is built once and applied many times. Replaces nothing in the reference
+ * directly: it is the device-resident form of the `events` argument of simulate_spectrum
+ * (simulation/simulate.py:134-148).
+ *
+ * Host arrays, each [n_offsets, n_events] row-major; rf_* and dt in float64 (callers holding float32
+ * events widen them, which is exact). reset_after_adc mirrors simulate_sequence's argument
+ * (simulate.py:80); the reference always simulates with True.
+ */
+typedef struct bmc_plan bmc_plan;
+int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
+                    const double* rf_phase, const double* rf_freq, int reset_after_adc, bmc_plan** out);
+int bmc_plan_destroy(bmc_plan* plan);
+/* Statistics of a plan: ops and the number of events whose propagator is built per offset row. */
+int bmc_plan_stats(const bmc_plan* plan, int64_t* n_ops, int64_t* n_built, int64_t* n_applied);
+
+/*
+ * Forward simulation with forward-mode tangents: for every voxel v and offset row o, the final state
+ * `last_M` of the event scan and, for each tangent p, its derivative w.r.t. packed parameter slot
+ * tangent_slot[p], times tangent_scale[p].
+ *
+ * Replaces: simulate_spectrum / simulate_voxels (simulation/simulate.py:134-148, voxels.py:89-129) for
+ * the state, and jax.jacfwd of the residual (fitting/lm.py:39, fitting/joint.py:300) for the tangents.
+ *
+ *   params_dev   [V, 13 + 5 n_cest]          packed records
+ *   m0_dev       [V, N]                      initial state per voxel (M_init)
+ *   dm0_dev      [V, P, N] or NULL           derivative of M_init per tangent (NULL = zero)
+ *   tangent_slot [P] host                    packed slot index, or -1 for "no generator dependence"
+ *   tangent_scale[P] host                    multiplier of tangent p (the FitSpec scale)
+ *   m_out_dev    [V, O, N] or NULL           final states
+ *   jac_out_dev  [V, O, P] or NULL           d state[jac_component] / d x_p
+ * NaN semantics follow the reference: an event whose reference squaring count exceeds 16
+ * (simulation/expm.py:83) makes the state NaN until the next ADC reset.
+ */
+int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                 const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
+                 const double* tangent_scale, int jac_component, void* m_out_dev, void* jac_out_dev,
+                 void* stream);
+
+/*
+ * Batched bounded Levenberg-Marquardt, one independent problem per voxel, every iteration on the GPU:
+ * simulate with tangents at x, float64 normal equations J^T J, J^T r with the damped solve
+ * A = H + lam (diag H + 1e-9 I) (free mask: J * free, A += diag(1 - free)), clip to [lo, hi], a primal
+ * simulation at the trial point, accept iff strictly better (lam x 0.5) else reject (lam x 3).
+ *
+ * Replaces: levenberg_marquardt vmapped by fit_batch (fitting/lm.py:20-67, independent.py:118-128) and
+ * fit_joint's non-coupled path (joint.py:359-374).
+ *
+ *   x_dev        [V, P] float64 in/out       scaled start values in, fitted scaled values out
+ *   lo, hi       [P] host float64            bounds, scaled units
+ *   z_meas_dev   [V, O] dtype                measured Z (residual r = Z_sim[:, jac_component] - z_meas)
+ *   free_dev     [V, P] float64 or NULL      1 free / 0 held (NULL = all free, arithmetic unchanged)
+ *   cost_traj_dev[V, n_iter] float64 or NULL cost after every iteration (the reference's C_traj)
+ *   x_traj_dev   [V, n_iter, P] float64 or NULL
+ *   flags        BMC_LM_EARLY_EXIT: stop a problem once it has converged (never the reference's
+ *                behaviour, which runs every iteration); BMC_LM_GRAPH: replay the iteration as a
+ *                captured hipGraph.
+ *   n_iter_done_dev [V] int32 or NULL        iterations each problem ran
+ */
+#define BMC_LM_EARLY_EXIT 1u
+#define BMC_LM_GRAPH 2u
+int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+               const void* m0_dev, int n_params, const int32_t* slot, const double* scale, const double* lo,
+               const double* hi, int jac_component, const void* z_meas_dev, double* x_dev, int n_iter,
+               double lam0, const double* free_dev, unsigned flags, double early_ftol, double early_rms,
+               double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BMC_HIP_H */
