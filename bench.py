@@ -1,0 +1,176 @@
+"""Benchmark of the simulate-and-fit hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--voxels V]
+
+Workload: the pulsed-train forward + Jacobian configuration -- 4 pools (water, amide +3.5 ppm, amine
++2.0 ppm, SuperLorentzian MT at -2.5 ppm), 50 Gaussian pulses x 200 steps of 0.1 ms with 20 ms gaps,
+a spoiler and a 3 s recovery (10,053 events per offset), 61 offsets, 65,536 voxels with seeded
+uniform / log-uniform parameters, 5 free values (k_amide, f_amide, water R1, water R2, B0 shift),
+float32 inputs and outputs. One step = the final state and the 5-column Jacobian of every voxel and
+offset (bmc_simulate). The metric is Z-spectra (forward + Jacobian) per second.
+
+Prints ONE JSON line. `roofline` is the dominant kernel's executed float64 flops (bmc_jax._flops,
+which replays the kernel's per-event Taylor degree and squaring count on a sample of voxels) over its
+measured launch time, against MI355X's FP64 vector peak (the kernel computes in float64).
+`cpu_baseline` times the reference-equivalent C oracle (one float64 Pade exponential per event, no
+reuse, forward-mode tangents, OpenMP over offsets) on this host's cores on one voxel's spectrum.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "bmc-jax_amd")]
+
+FREE = [("cest_k", 0), ("cest_f", 0), ("water_r1", None), ("water_r2", None), ("b0_inhom", None)]
+PEAK_FP64_TFLOPS = 78.6
+
+
+def workload(n_vox, seed=2):
+    """Packed float32 parameters [V, 23], M0 [V, 10], events of the pulsed train."""
+    from bmc_jax.sequences import pulsed_train
+    rng = np.random.default_rng(seed)
+    nc = 2
+    V = n_vox
+    prm = np.zeros((V, 13 + 5 * nc))
+    k = np.exp(rng.uniform(np.log(5.0), np.log(5000.0), (V, 2)))
+    f = rng.uniform(1e-5, 0.02, (V, 2))
+    prm[:, 0] = 1.0 / rng.uniform(0.8, 3.0, V)        # water R1
+    prm[:, 1] = 1.0 / rng.uniform(0.04, 0.15, V)      # water R2
+    prm[:, 2] = 1.0                                   # water f
+    prm[:, 3:9] = [30.0, 0.1, -2.5, 1.0, 1e5, 2]      # MT k, f, dw, r1, r2 (T2 10 us), SuperLorentzian
+    prm[:, 9], prm[:, 10] = 3.0, 267.5153             # B0, gamma
+    prm[:, 11] = rng.uniform(-0.3, 0.3, V)            # B0 shift [ppm]
+    prm[:, 12] = 1.0
+    prm[:, 13:15] = k
+    prm[:, 15:17] = f
+    prm[:, 17:19] = [3.5, 2.0]                        # dw
+    prm[:, 19:21] = [1.0 / 1.3, 1.0 / 1.3]            # r1
+    prm[:, 21:23] = [10.0, 20.0]                      # r2 (T2 0.1 s, 0.05 s)
+    prm = prm.astype(np.float32)
+    N = 3 * (1 + nc) + 1
+    m0 = np.zeros((V, N), np.float32)
+    m0[:, 6] = prm[:, 2]
+    m0[:, 7:9] = prm[:, 15:17]
+    m0[:, 9] = prm[:, 4]
+    ev = pulsed_train(np.linspace(-6.0, 6.0, 61), b1_peak_uT=2.0, n_pulses=50, pulse_s=0.02, n_samples=200,
+                      gap_s=0.02, recovery_s=3.0, b0_T=3.0)
+    return prm, m0, ev, nc
+
+
+def cpu_baseline(prm, m0, ev, nc, slots, budget_s=20.0):
+    """The C oracle on this host's cores: as many offsets of voxel 0 as fit the budget (all 61 if they do)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_c as OC
+    threads = os.cpu_count() or 1
+    ev64 = {k: np.asarray(getattr(ev, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    one = {k: v[:1] for k, v in ev64.items()}
+    t0 = time.perf_counter()
+    OC.simulate(prm[:1].astype(np.float64), nc, one, m0[:1].astype(np.float64), slots=slots, threads=1)
+    per_row = time.perf_counter() - t0
+    O = ev64["dt"].shape[0]
+    n_rows = int(min(O, max(1, budget_s * threads / max(per_row, 1e-9))))
+    sub = {k: v[:n_rows] for k, v in ev64.items()}
+    t0 = time.perf_counter()
+    OC.simulate(prm[:1].astype(np.float64), nc, sub, m0[:1].astype(np.float64), slots=slots, threads=threads)
+    wall = time.perf_counter() - t0
+    spectra_per_s = (n_rows / O) / wall
+    return dict(value=spectra_per_s, unit="Z-spectra/s (fwd+Jac)", cores=threads, kind="port",
+                sample=f"1 voxel x {n_rows} of {O} offsets x 10,053 events, 5 tangents, float64 Pade-13 per event "
+                       f"(no reuse), {wall:.1f} s wall on {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--voxels", type=int, default=65536)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--flop-sample", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (voxels are independent: "
+              "N GPUs would each run 1/N of the batch)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bmc_jax import _engine, _flops, _native
+    from bmc_jax.simulation.events import SimEvent
+
+    _native.torch_cuda()
+    prm, m0, ev, nc = workload(args.voxels)
+    slots = [_native.slot_index(f, i, nc) for f, i in FREE]
+    scales = [100.0, 0.001, 1.0, 10.0, 0.1]
+    plan = _engine.plan_for(SimEvent(*ev))
+    prm_d = _native.device_array(prm, torch.float32)
+    m0_d = _native.device_array(m0, torch.float32)
+    jc = 2 * (1 + nc)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        return _native.simulate(plan, prm_d, nc, _native.BMC_F32, m0_d, slots=slots, scales=scales, jc=jc)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        m, J = step()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms = 1e3 * wall / args.steps
+    V = args.voxels
+    spectra_per_s = V / (wall / args.steps)
+    assert torch.isfinite(J).all().item() and torch.isfinite(m).all().item()
+
+    # roofline: executed flops of one launch from a sample of voxels
+    ops, rp = plan.program()
+    rng = np.random.default_rng(0)
+    sample = rng.choice(V, size=min(args.flop_sample, V), replace=False)
+    ev_np = {k: np.asarray(getattr(ev, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    per_vox = _flops.executed_flops(prm[sample].astype(np.float64), nc, ev_np, ops, rp, slots, pt=1)
+    flops_launch = float(per_vox.mean()) * V
+    avg_launch_s = 1e-3 * float(np.mean(launch_ms))
+    achieved = flops_launch / avg_launch_s / 1e12
+    # reference-equivalent flops (every event its own Pade-13-class exponential with 5 tangents)
+    stats = plan.stats()
+
+    out = {
+        "metric": "Z-spectra/s (forward + 5-column Jacobian), pulsed train",
+        "value": spectra_per_s,
+        "unit": "Z-spectra/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded parameter draws; no measured data)",
+        "config": {"workload": "pulsed train fwd+Jac: 4 pools (water, amide, amine, SL MT), 50 Gaussian pulses x 200 "
+                               "steps + gaps, spoiler, 3 s recovery (10,053 events), 61 offsets, "
+                               f"{V} voxels, 5 free values, float32 I/O",
+                   "voxels": V, "offsets": 61, "events_per_offset": int(ev_np['dt'].shape[1]), "free_values": 5,
+                   "io_dtype": "float32", "plan_built_events_per_offset": stats["n_built"] / 61},
+        "roofline": {"bound": "fp64 VALU", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops_launch, "avg_launch_ms": 1e3 * avg_launch_s},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prm, m0, ev, nc, slots)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

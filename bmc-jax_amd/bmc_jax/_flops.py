@@ -1,0 +1,155 @@
+"""Algorithmic FLOP counts of the simulate kernel (csrc/bmc_sim.hip), for roofline reporting.
+
+`executed_flops` replays, on the host, the kernel's per-event choice of Taylor degree m and squaring
+count s (the same 1-norm and theta_m table) and counts the useful float64 operations of every step:
+  generator apply X v       29 NC + 27 flops (13 NC + 13 FMA, 3 NC + 1 multiplies)
+  tangent apply dX v        per slot kind (below), plus X dv
+  Taylor term bookkeeping   2 N per vector (scale by 1/k, accumulate)
+  dense product (column)    2 N^2 (1 + 2 PT) per column, N + 1 useful columns
+  apply to the state        (N + 1)(N + 3 PT N) + N (1 + PT) N adds of the team reduction
+Only the N + 1 columns of a propagator count, not the idle lanes of a 16-lane team.
+`reference_flops` is the survey's model of the reference's per-event work (every event its own
+Pade exponential with its tangents, reference float32 degree/squaring rule).
+"""
+import math
+
+import numpy as np
+
+THETA = np.array([0.0, 2.22e-16, 2.581e-08, 1.386e-05, 0.0003397, 0.002401, 0.009066, 0.02384, 0.04991, 0.08958,
+                  0.1442, 0.2142, 0.2996, 0.3998, 0.5139, 0.6411, 0.7803, 0.9305, 1.091, 1.26, 1.438, 1.624, 1.816,
+                  2.015, 2.219, 2.429, 2.643, 2.861, 3.084, 3.31, 3.54])
+M_MAX = 30
+
+# tangent kinds by packed slot (csrc/bmc_sim.hip tan_kind_of_slot)
+_DX_FLOPS_SCALAR = {0: 4, 1: 4, 2: 0, 3: 8, 4: 6, 5: 2, 6: 4, 7: 2, 8: 0}
+
+
+def dx_flops(slot, nc):
+    if slot < 0:
+        return 0
+    if slot in _DX_FLOPS_SCALAR:
+        return _DX_FLOPS_SCALAR[slot]
+    if slot in (9, 10):                # b0, gamma
+        return 2 * (3 + 3 * nc)
+    if slot == 11:                     # b0_inhom
+        return 2 * (3 + 2 * nc)
+    if slot == 12:                     # rel_b1
+        return 2 * (5 + 4 * nc)
+    kind = (slot - 13) // nc           # cest_k, cest_f, cest_dw, cest_r1, cest_r2
+    return (24, 14, 4, 4, 4)[kind]
+
+
+def _gen_l1(prm, nc, dt, amp, cph, sph, fr):
+    """The kernel's augmented 1-norm per (voxel, event) (build_gen), vectorised. prm [V, S]; events 1-D."""
+    r1w, r2w, kmt, fmt, dwmt, r1mt, r2mt = (prm[:, i][:, None] for i in (0, 1, 3, 4, 5, 6, 7))
+    ls = prm[:, 8][:, None]
+    b0, gam, b0i, relb1 = (prm[:, i][:, None] for i in (9, 10, 11, 12))
+    k = prm[:, 13:13 + nc][:, :, None]
+    f = prm[:, 13 + nc:13 + 2 * nc][:, :, None]
+    dw = prm[:, 13 + 2 * nc:13 + 3 * nc][:, :, None]
+    r1c = prm[:, 13 + 3 * nc:13 + 4 * nc][:, :, None]
+    r2c = prm[:, 13 + 4 * nc:13 + 5 * nc][:, :, None]
+    L = gam * 1e6 * b0
+    ks = (prm[:, 13:13 + nc] * prm[:, 13 + nc:13 + 2 * nc]).sum(1)[:, None]
+    kwmt = kmt * fmt
+    w1 = 2 * np.pi * amp[None, :] * relb1
+    w1x, w1y = w1 * cph[None, :], w1 * sph[None, :]
+    eff = 2 * np.pi * fr[None, :] - L * b0i * 1e-6
+    # the MT saturation term only matters through |dmt| in the norm; w_mt >= 0 adds to it
+    from .simulation.matrix import lorentzian, super_lorentzian
+    wmt = np.zeros_like(eff)
+    on = (fmt[:, 0] > 1e-9) & (ls[:, 0] != 0)
+    if on.any():
+        for v in np.flatnonzero(on):
+            off = eff[v] - dwmt[v, 0] * 1e-6 * L[v, 0]
+            t2 = 1.0 / r2mt[v, 0]
+            for j in np.flatnonzero(w1[v] != 0):
+                g = lorentzian(off[j], t2) if ls[v, 0] == 1 else super_lorentzian(off[j], t2, L[v, 0] * 1e-6)
+                wmt[v, j] = w1[v, j] ** 2 * g
+    axx, azz, dmt = -r2w - ks, -r1w - ks - kwmt, -r1mt - kmt - wmt
+    skw = np.abs(prm[:, 13:13 + nc] * prm[:, 13 + nc:13 + 2 * nc]).sum(1)[:, None]
+    n1 = np.maximum.reduce([np.abs(axx) + np.abs(eff) + np.abs(w1y) + skw, np.abs(eff) + np.abs(axx) + np.abs(w1x) + skw,
+                            np.abs(w1y) + np.abs(w1x) + np.abs(azz) + skw + np.abs(kwmt), np.abs(kmt) + np.abs(dmt),
+                            (np.abs(r1w) + np.abs(r1mt * fmt) + np.abs(r1c * f).sum(1)) * np.ones_like(eff)])
+    if nc:
+        oi = dw * 1e-6 * L[:, :, None] - eff[:, None, :]
+        bi, ci = -r2c - k, -r1c - k
+        p1 = (np.abs(k) + np.abs(bi) + np.abs(oi) + np.abs(w1y[:, None, :])).max(1)
+        p2 = (np.abs(k) + np.abs(oi) + np.abs(bi) + np.abs(w1x[:, None, :])).max(1)
+        p3 = (np.abs(k) + np.abs(w1y[:, None, :]) + np.abs(w1x[:, None, :]) + np.abs(ci)).max(1)
+        n1 = np.maximum.reduce([n1, p1, p2, p3])
+    return n1 * dt[None, :]
+
+
+def degree_and_squarings(l1):
+    l1 = np.asarray(l1, np.float64)
+    m = np.searchsorted(THETA[1:M_MAX + 1], l1, side="left") + 1     # smallest m with theta_m >= l1
+    m = np.clip(m, 1, M_MAX)
+    s = np.where(l1 <= THETA[M_MAX], 0, np.ceil(np.log2(np.maximum(l1, 1e-300) / THETA[M_MAX])))
+    s = np.maximum(s, 0).astype(np.int64)
+    big = s > 0
+    while True:
+        bad = big & (l1 * np.ldexp(1.0, -s) > THETA[M_MAX])
+        if not bad.any():
+            break
+        s = s + bad
+    m = np.where(big, M_MAX, m)
+    return m.astype(np.int64), s
+
+
+def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1):
+    """Useful float64 flops of one bmc_simulate call, per voxel of prm [V, S] (summed over offsets and passes).
+
+    events: dict/SimEvent of (O, T) arrays; ops/row_ptr: the plan's program; slots: tangent slots.
+    """
+    ev = {k: np.asarray(getattr(events, k) if not isinstance(events, dict) else events[k])
+          for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    N = 3 * (1 + nc) + 1
+    NA = N + 1
+    FX = 29 * nc + 27
+    P = len(slots)
+    passes = [slots[i:i + pt] for i in range(0, P, pt)] if P else [[]]
+    V = prm.shape[0]
+    total = np.zeros(V)
+    O = ev["dt"].shape[0]
+    for o in range(O):
+        for oi in range(row_ptr[o], row_ptr[o + 1]):
+            kind, e0, ln, cnt = (int(v) for v in ops[oi])
+            sel = np.arange(e0, e0 + ln)
+            dt = ev["dt"][o, sel].astype(np.float64)
+            live = dt > 1e-9
+            if not live.any():
+                continue
+            sel = sel[live]
+            l1 = _gen_l1(prm, nc, dt[live], ev["rf_amp"][o, sel].astype(np.float64),
+                         np.cos(ev["rf_phase"][o, sel].astype(np.float64)),
+                         np.sin(ev["rf_phase"][o, sel].astype(np.float64)), ev["rf_freq"][o, sel].astype(np.float64))
+            m, s = degree_and_squarings(l1)
+            for ps in passes:
+                T = len(ps)
+                fdx = sum(dx_flops(sl, nc) for sl in ps)
+                term = FX + T * FX + fdx + 2 * N * (1 + T)
+                mul = NA * 2 * N * N * (1 + 2 * T)
+                app = NA * (N + 3 * T * N) + N * (1 + T) * (NA - 1)
+                if kind == 0:
+                    vec = (s == 0)
+                    f = np.where(vec, m * term, NA * m * term + s * mul + app)
+                    total += f.sum(axis=1)
+                else:
+                    f = NA * m * term + s * mul + mul
+                    total += f.sum(axis=1) + cnt * app
+    return total
+
+
+def reference_flops(n_aug, P, m_ref, s_ref):
+    """The survey's model of the reference's per-event work: expm + P tangents + apply (flops)."""
+    km = {3: 2, 5: 3, 7: 4, 9: 5, 13: 6}[m_ref]
+    n3, n2 = n_aug ** 3, n_aug ** 2
+    return 2 * n3 * (km + s_ref) + 8.0 / 3.0 * n3 + P * (4 * n3 * (km + s_ref) + 4 * n3) + 2 * n2 + 4 * P * n2
+
+
+def reference_rule(l1):
+    """Reference float32 Pade degree and squarings from the unscaled 1-norm (expm.py:73-77)."""
+    s = 0 if l1 <= 0 else max(0, math.floor(math.log2(l1 / 3.925724783138660)))
+    m = 3 if l1 < 0.4258730016922831 else 5 if l1 < 1.880152677804762 else 7
+    return m, s

@@ -61,14 +61,20 @@ def lib():
         L.bmc_device_count.restype = ctypes.c_int
         L.bmc_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, ip32, dp, dp, dp, dp, ctypes.c_int,
                                       ctypes.POINTER(vp)]
+        L.bmc_plan_create_ex.argtypes = [ctypes.c_int, ctypes.c_int, ip32, dp, dp, dp, dp, ctypes.c_int,
+                                         ctypes.c_int, ctypes.POINTER(vp)]
+        L.bmc_simulate_trajectory.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         L.bmc_plan_destroy.argtypes = [vp]
         L.bmc_plan_stats.argtypes = [vp, i64p, i64p, i64p]
+        L.bmc_plan_program.argtypes = [vp, ctypes.c_int64, ip32, ip32]
         L.bmc_simulate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32, dp,
                                    ctypes.c_int, vp, vp, vp]
         L.bmc_lm_fit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ip32, dp, dp, dp,
                                  ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_uint,
                                  ctypes.c_double, ctypes.c_double, vp, vp, vp, vp]
-        for name in ("bmc_plan_create", "bmc_plan_destroy", "bmc_plan_stats", "bmc_simulate", "bmc_lm_fit"):
+        for name in ("bmc_plan_create", "bmc_plan_create_ex", "bmc_plan_destroy", "bmc_plan_stats", "bmc_plan_program",
+                     "bmc_simulate",
+                     "bmc_simulate_trajectory", "bmc_lm_fit"):
             getattr(L, name).restype = ctypes.c_int
         if L.bmc_abi_version() != 1:
             raise NativeError("libbmc_hip.so ABI version mismatch")
@@ -102,7 +108,7 @@ def _dp(a):
 class Plan:
     """A sequence compiled for the device (bmc_plan_create): events [O, T] and the row programs."""
 
-    def __init__(self, type_, dt, rf_amp, rf_phase, rf_freq, reset_after_adc=True):
+    def __init__(self, type_, dt, rf_amp, rf_phase, rf_freq, reset_after_adc=True, compress=True):
         L = lib()
         torch_cuda()
         t = np.ascontiguousarray(np.asarray(type_), dtype=np.int32)
@@ -115,9 +121,10 @@ class Plan:
                 raise ValueError("event fields disagree in shape")
         self.n_offsets, self.n_events = t.shape
         h = ctypes.c_void_p()
-        check(L.bmc_plan_create(self.n_offsets, self.n_events, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                *[_dp(a) for a in self._arrays[1:]], 1 if reset_after_adc else 0, ctypes.byref(h)),
-              "bmc_plan_create")
+        check(L.bmc_plan_create_ex(self.n_offsets, self.n_events, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                   *[_dp(a) for a in self._arrays[1:]], 1 if reset_after_adc else 0,
+                                   1 if compress else 0, ctypes.byref(h)), "bmc_plan_create_ex")
+        self.compressed = bool(compress)
         self._h = h
         self._arrays = None
 
@@ -129,6 +136,15 @@ class Plan:
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check(lib().bmc_plan_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "bmc_plan_stats")
         return dict(n_ops=a.value, n_built=b.value, n_applied=c.value)
+
+    def program(self):
+        """(ops [n_ops, 4] int32, row_ptr [O + 1] int32): the compiled row programs."""
+        n = self.stats()["n_ops"]
+        ops = np.zeros((max(n, 1), 4), np.int32)
+        rp = np.zeros(self.n_offsets + 1, np.int32)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        check(lib().bmc_plan_program(self._h, max(n, 1), ip(ops), ip(rp)), "bmc_plan_program")
+        return ops[:n], rp
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -167,3 +183,36 @@ def simulate(plan: Plan, params, n_cest: int, dtype_code: int, m0, dm0=None, slo
     check(L.bmc_simulate(plan.handle, V, n_cest, dtype_code, ptr(params), ptr(m0), ptr(dm0), P, sl, sc, jc,
                          ptr(m_out), ptr(jac), stream_ptr()), "bmc_simulate")
     return m_out, jac
+
+
+def simulate_trajectory(plan: Plan, params, n_cest: int, dtype_code: int, m0):
+    """bmc_simulate_trajectory: the state after every event [V, O, T, N] (device tensor)."""
+    torch = torch_cuda()
+    V = int(params.shape[0])
+    N = 3 * (1 + n_cest) + 1
+    tdt = torch.float64 if dtype_code == BMC_F64 else torch.float32
+    out = torch.empty((V, plan.n_offsets, plan.n_events, N), dtype=tdt, device="cuda")
+    check(lib().bmc_simulate_trajectory(plan.handle, V, n_cest, dtype_code, ctypes.c_void_p(params.data_ptr()),
+                                        ctypes.c_void_p(m0.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                        stream_ptr()), "bmc_simulate_trajectory")
+    return out
+
+
+def lm_fit(plan: Plan, params, n_cest: int, dtype_code: int, m0, slots, scales, lo, hi, jc, z_meas, x0, n_iter,
+           lam0=1e-2, free=None, flags=0, early_ftol=0.0, early_rms=0.0, want_traj=False):
+    """bmc_lm_fit on device tensors. x0 [V, P] float64 (scaled). Returns (x, cost_traj, x_traj, n_done)."""
+    torch = torch_cuda()
+    V, P = int(x0.shape[0]), len(slots)
+    x = x0.to(device="cuda", dtype=torch.float64).contiguous().clone()
+    ctraj = torch.empty((V, n_iter), dtype=torch.float64, device="cuda") if want_traj else None
+    xtraj = torch.empty((V, n_iter, P), dtype=torch.float64, device="cuda") if want_traj else None
+    ndone = torch.empty((V,), dtype=torch.int32, device="cuda")
+    arr = lambda ct, vals: (ct * max(len(vals), 1))(*vals)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    fr = None if free is None else free.to(device="cuda", dtype=torch.float64).contiguous()
+    check(lib().bmc_lm_fit(plan.handle, V, n_cest, dtype_code, ptr(params), ptr(m0), P,
+                           arr(ctypes.c_int32, [int(s) for s in slots]), arr(ctypes.c_double, [float(s) for s in scales]),
+                           arr(ctypes.c_double, [float(v) for v in lo]), arr(ctypes.c_double, [float(v) for v in hi]),
+                           jc, ptr(z_meas), ptr(x), int(n_iter), float(lam0), ptr(fr), int(flags), float(early_ftol),
+                           float(early_rms), ptr(ctraj), ptr(xtraj), ptr(ndone), stream_ptr()), "bmc_lm_fit")
+    return x, ctraj, xtraj, ndone

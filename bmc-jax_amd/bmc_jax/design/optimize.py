@@ -1,0 +1,80 @@
+"""A protocol with a lower Cramer-Rao bound, by gradient descent (src/bmc_jax/design/optimize.py:31-113).
+
+cost(d) = mean over truths of sum over targets of var_j / x_j^2, with d = lo + (hi - lo) sigmoid(u) and
+Adam on u. `build(d)` is arbitrary caller code here (NumPy, not a traced function), so the gradient
+of the cost with respect to d is taken by central differences of the GPU-evaluated cost, one pair of
+Fisher evaluations per design entry.
+"""
+from dataclasses import dataclass
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from .crlb import crlb, fisher_information
+
+
+@dataclass(frozen=True)
+class DesignResult:
+    design: np.ndarray
+    start: np.ndarray
+    cost: np.ndarray
+    targets: tuple
+    truths: np.ndarray
+    rel_sd_start: np.ndarray
+    rel_sd_end: np.ndarray
+
+
+def optimize_design(build: Callable, spec, d0, lo, hi, sigma: float, truths, targets: Optional[Sequence[str]] = None,
+                    n_steps: int = 200, lr: float = 0.05, callback: Optional[Callable] = None,
+                    fd_step: float = 1e-4) -> DesignResult:
+    """Lower the bound of the target parameters by changing the design (optimize.py:43-113)."""
+    names = [s.name for s in spec.slots]
+    targets = tuple(names if targets is None else targets)
+    idx = np.asarray([names.index(t) for t in targets])
+    truths = np.atleast_2d(np.asarray(truths, float))
+    d0, lo, hi = (np.asarray(a, float) for a in (d0, lo, hi))
+    if not (d0.shape == lo.shape == hi.shape) or np.any(d0 <= lo) or np.any(d0 >= hi):
+        raise ValueError("d0, lo and hi need the same shape, with lo < d0 < hi")
+
+    def cost_of(d):
+        exp = build(d)
+        total = 0.0
+        for x in truths:
+            F, _ = fisher_information(exp, spec, x, sigma)
+            var = np.diag(np.linalg.inv(F))[idx]
+            total += float(np.sum(var / np.asarray(x)[idx] ** 2))
+        return total / len(truths)
+
+    sig = lambda u: 1.0 / (1.0 + np.exp(-u))
+    to_d = lambda u: lo + (hi - lo) * sig(u)
+    p0 = (d0 - lo) / (hi - lo)
+    u = np.log(p0 / (1.0 - p0))
+
+    def value_and_grad(u):
+        c = cost_of(to_d(u))
+        g = np.zeros_like(u)
+        for j in range(u.size):
+            e = np.zeros_like(u)
+            e[j] = fd_step
+            g[j] = (cost_of(to_d(u + e)) - cost_of(to_d(u - e))) / (2 * fd_step)
+        return c, g
+
+    m, v, b1, b2, eps = np.zeros_like(u), np.zeros_like(u), 0.9, 0.999, 1e-8
+    history = []
+    for step in range(n_steps):
+        c, g = value_and_grad(u)
+        if not (np.isfinite(c) and np.all(np.isfinite(g))):
+            raise FloatingPointError(f"cost or gradient not finite at step {step}; try float64 or a smaller lr")
+        history.append(float(c))
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g ** 2
+        u = u - lr * (m / (1 - b1 ** (step + 1))) / (np.sqrt(v / (1 - b2 ** (step + 1))) + eps)
+        if callback is not None:
+            callback(step, float(c), np.asarray(to_d(u)))
+    design = np.asarray(to_d(u), float)
+    history.append(float(cost_of(design)))
+
+    def rel(d):
+        exp = build(np.asarray(d))
+        return np.array([[crlb(exp, spec, sigma, x).rel_sd[names.index(t)] for t in targets] for x in truths])
+    return DesignResult(design, d0, np.asarray(history), targets, truths, rel(d0), rel(design))

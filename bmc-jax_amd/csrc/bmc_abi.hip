@@ -109,6 +109,12 @@ int bmc_device_count(void) {
 
 int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
                     const double* rf_phase, const double* rf_freq, int reset_after_adc, bmc_plan** out) {
+  return bmc_plan_create_ex(n_offsets, n_events, type, dt, rf_amp, rf_phase, rf_freq, reset_after_adc, 1, out);
+}
+
+int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
+                       const double* rf_phase, const double* rf_freq, int reset_after_adc, int compress,
+                       bmc_plan** out) {
   if (!out || n_offsets <= 0 || n_events < 0 || (n_events > 0 && (!type || !dt || !rf_amp || !rf_phase || !rf_freq))) {
     set_error("bmc_plan_create: bad arguments");
     return BMC_EINVAL;
@@ -131,7 +137,13 @@ int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const doub
       ev[k] = Ev{dt[k], rf_amp[k], cos(rf_phase[k]), sin(rf_phase[k]), rf_freq[k], type[k], 0};
     }
     row_ptr[o] = (int32_t)ops.size();
-    compile_row(row, ops, built, applied);
+    if (compress) {
+      compile_row(row, ops, built, applied);
+    } else if (T > 0) {
+      ops.push_back(Op{OP_SEQ, 0, T, 1});
+      built += T;
+      applied += T;
+    }
   }
   row_ptr[O] = (int32_t)ops.size();
   bmc_plan* pl = new bmc_plan();
@@ -139,9 +151,12 @@ int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const doub
   p.O = O;
   p.T = T;
   p.reset_after_adc = reset_after_adc ? 1 : 0;
+  p.compressed = compress ? 1 : 0;
   p.n_ops = (int64_t)ops.size();
   p.n_built = built;
   p.n_applied = applied;
+  p.h_ops = ops;
+  p.h_row_ptr = row_ptr;
   auto fail = [&](hipError_t e) {
     set_error(std::string("bmc_plan_create: ") + hipGetErrorString(e));
     bmc_plan_destroy(pl);
@@ -180,6 +195,22 @@ int bmc_plan_stats(const bmc_plan* plan, int64_t* n_ops, int64_t* n_built, int64
   if (n_ops) *n_ops = plan->p.n_ops;
   if (n_built) *n_built = plan->p.n_built;
   if (n_applied) *n_applied = plan->p.n_applied;
+  return BMC_OK;
+}
+
+int bmc_plan_program(const bmc_plan* plan, int64_t capacity, int32_t* ops_out, int32_t* row_ptr_out) {
+  if (!plan || !ops_out || !row_ptr_out || capacity < plan->p.n_ops) {
+    set_error("bmc_plan_program: null argument or capacity below n_ops");
+    return BMC_EINVAL;
+  }
+  const Plan& p = plan->p;
+  for (size_t i = 0; i < p.h_ops.size(); i++) {
+    ops_out[4 * i + 0] = p.h_ops[i].kind;
+    ops_out[4 * i + 1] = p.h_ops[i].e0;
+    ops_out[4 * i + 2] = p.h_ops[i].len;
+    ops_out[4 * i + 3] = p.h_ops[i].count;
+  }
+  for (size_t i = 0; i < p.h_row_ptr.size(); i++) row_ptr_out[i] = p.h_row_ptr[i];
   return BMC_OK;
 }
 
@@ -264,5 +295,25 @@ extern "C" int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int 
   a.PT = tangents_per_pass(want_tan);
   a.n_pass = want_tan ? (want_tan + a.PT - 1) / a.PT : 1;
   if (!want_tan) a.P = n_tangents;  // seeds unused
+  return launch_sim(n_cest, a, (hipStream_t)stream);
+}
+
+extern "C" int bmc_simulate_trajectory(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                                       const void* m0_dev, void* traj_out_dev, void* stream) {
+  SimArgs a;
+  int rc = fill_sim_args(plan, n_voxels, n_cest, dtype, params_dev, m0_dev, nullptr, 0, nullptr, nullptr, 0, a);
+  if (rc != BMC_OK) return rc;
+  if (plan->p.compressed) {
+    set_error("bmc_simulate_trajectory: the plan must be created with compress = 0");
+    return BMC_EINVAL;
+  }
+  if (!traj_out_dev) {
+    set_error("bmc_simulate_trajectory: null output");
+    return BMC_EINVAL;
+  }
+  if (n_voxels == 0) return BMC_OK;
+  a.traj = traj_out_dev;
+  a.PT = 0;
+  a.n_pass = 1;
   return launch_sim(n_cest, a, (hipStream_t)stream);
 }

@@ -75,6 +75,7 @@ struct SimArgs {
   int jac_dtype;
   const int32_t* active;  // [V] or null: skip voxels whose flag is 0
   double nan_l1;          // reference NaN threshold on the augmented 1-norm
+  void* traj;             // [V, O, T, N] dtype or null: state after every event (SEQ-only plans)
 };
 
 struct Plan {
@@ -83,7 +84,9 @@ struct Plan {
   Op* d_ops = nullptr;
   int32_t* d_row_ptr = nullptr;
   int64_t n_ops = 0, n_built = 0, n_applied = 0;
-  int max_events_per_row = 0;
+  int compressed = 1;
+  std::vector<Op> h_ops;          // host copy of the program
+  std::vector<int32_t> h_row_ptr; // host copy of the row offsets
 };
 
 void set_error(const std::string& msg);

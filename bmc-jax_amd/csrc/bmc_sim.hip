@@ -35,7 +35,7 @@ __constant__ double kTheta[41] = {0.0,      2.22e-16, 2.581e-08, 1.386e-05, 0.00
                                   1.624,    1.816,    2.015,     2.219,     2.429,     2.643,    2.861,
                                   3.084,    3.31,     3.54,      3.772,     4.008,     4.245,    4.486,
                                   4.728,    4.973,    5.219,     5.468,     5.717,     5.969};
-constexpr int kMMax = 24;  // above theta_24 another squaring is cheaper than more terms
+constexpr int kMMax = 30;  // the most accurate cap for the squaring path (fewest squarings before the rounding of the series takes over)
 
 __device__ __forceinline__ double load_real(const void* p, long i, int dtype) {
   return dtype == BMC_F64 ? static_cast<const double*>(p)[i] : (double)static_cast<const float*>(p)[i];
@@ -736,6 +736,11 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
 #pragma unroll
             for (int p = 0; p < PTA; p++) dM[p][i] = 0.0;
           }
+        }
+        if (a.traj && !dead && q == 0 && lane == 0) {  // the trajectory: last_M after every event
+          const long base = (((long)v * a.O + o) * a.T + ei) * N;
+#pragma unroll
+          for (int i = 0; i < N; i++) store_real(a.traj, base + i, M[i], a.dtype);
         }
         if (e.type == BMC_EV_ADC && a.reset_after_adc) pending = true;  // simulate.py:105-114
       }

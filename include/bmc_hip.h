@@ -74,9 +74,17 @@ int bmc_device_count(void);
 typedef struct bmc_plan bmc_plan;
 int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
                     const double* rf_phase, const double* rf_freq, int reset_after_adc, bmc_plan** out);
+/* As bmc_plan_create; compress = 0 keeps every event in order as its own step (no chained runs), which
+ * bmc_simulate_trajectory needs. */
+int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
+                       const double* rf_phase, const double* rf_freq, int reset_after_adc, int compress,
+                       bmc_plan** out);
 int bmc_plan_destroy(bmc_plan* plan);
 /* Statistics of a plan: ops and the number of events whose propagator is built per offset row. */
 int bmc_plan_stats(const bmc_plan* plan, int64_t* n_ops, int64_t* n_built, int64_t* n_applied);
+/* The row programs: ops_out [n_ops, 4] = (kind 0 SEQ / 1 BLOCK, first event, length, repeat count),
+ * row_ptr_out [n_offsets + 1]. capacity is the number of ops ops_out can hold. */
+int bmc_plan_program(const bmc_plan* plan, int64_t capacity, int32_t* ops_out, int32_t* row_ptr_out);
 
 /*
  * Forward simulation with forward-mode tangents: for every voxel v and offset row o, the final state
@@ -100,6 +108,13 @@ int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, cons
                  const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
                  const double* tangent_scale, int jac_component, void* m_out_dev, void* jac_out_dev,
                  void* stream);
+
+/*
+ * The state after every event: simulate_sequence's return_full_trajectory=True (simulate.py:115-120,
+ * 128-129): traj_out_dev [V, O, T, N] in dtype. The plan must come from bmc_plan_create_ex(compress=0).
+ */
+int bmc_simulate_trajectory(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                            const void* m0_dev, void* traj_out_dev, void* stream);
 
 /*
  * Batched bounded Levenberg-Marquardt, one independent problem per voxel, every iteration on the GPU:

@@ -25,7 +25,7 @@
 #include <string.h>
 
 #define MAXN 21
-#define MAXP 10
+#define MAXP 16
 
 static int stride_of(int nc) { return 13 + 5 * nc; }
 
@@ -161,7 +161,7 @@ static const double B13[14] = {64764752532480000., 32382376266240000., 777177030
  * E = expm(X) by Higham (2005) Pade 3/5/7/9/13 scaling and squaring, with P forward-mode tangents
  * dE[p] = L(X, dX[p]). Returns the squaring count (the caller applies the reference NaN rule).
  */
-static int expm_dual(int n, const double* X, int P, const double* dX, double* E, double* dE) {
+static int expm_dual(int n, const double* X, int P, const double* dX, double* E, double* dE, double* ws) {
     static const double th[5] = {1.495585217958292e-2, 2.539398330063230e-1, 9.504178996162932e-1,
                                  2.097847961257068e0, 5.371920351148152e0};
     static const double b3[4] = {120., 60., 12., 1.};
@@ -182,8 +182,17 @@ static int expm_dual(int n, const double* X, int P, const double* dX, double* E,
     double sc = ldexp(1.0, -s);
     double A[MAXN * MAXN], A2[MAXN * MAXN], A4[MAXN * MAXN], A6[MAXN * MAXN], T[MAXN * MAXN];
     double U[MAXN * MAXN], V[MAXN * MAXN], W[MAXN * MAXN];
-    double dA[MAXP][MAXN * MAXN], dA2[MAXP][MAXN * MAXN], dA4[MAXP][MAXN * MAXN], dA6[MAXP][MAXN * MAXN];
-    double dU[MAXP][MAXN * MAXN], dV[MAXP][MAXN * MAXN], dW[MAXP][MAXN * MAXN];
+    /* tangent arrays live in the caller's workspace: 9 blocks of MAXP x MAXN^2 */
+    typedef double Blk[MAXN * MAXN];
+    Blk* dA = (Blk*)(ws);
+    Blk* dA2 = dA + MAXP;
+    Blk* dA4 = dA2 + MAXP;
+    Blk* dA6 = dA4 + MAXP;
+    Blk* dU = dA6 + MAXP;
+    Blk* dV = dU + MAXP;
+    Blk* dW = dV + MAXP;
+    Blk* dR = dW + MAXP;
+    Blk* dA8 = dR + MAXP;
     for (int i = 0; i < nn; i++) A[i] = X[i] * sc;
     for (int p = 0; p < P; p++) for (int i = 0; i < nn; i++) dA[p][i] = dX[p * nn + i] * sc;
     mm(n, A, A, A2);
@@ -202,7 +211,7 @@ static int expm_dual(int n, const double* X, int P, const double* dX, double* E,
     for (int p = 0; p < P; p++) { memset(dW[p], 0, sizeof(double) * nn); memset(dV[p], 0, sizeof(double) * nn); }
     if (m < 13) {
         const double* pw[5] = {0, A2, A4, A6, 0};
-        double A8[MAXN * MAXN], dA8[MAXP][MAXN * MAXN];
+        double A8[MAXN * MAXN];
         if (m == 9) {
             mm(n, A4, A4, A8);
             for (int p = 0; p < P; p++) { mm(n, dA4[p], A4, dA8[p]); mm_acc(n, A4, dA4[p], dA8[p]); }
@@ -248,7 +257,6 @@ static int expm_dual(int n, const double* X, int P, const double* dX, double* E,
     for (int i = 0; i < nn; i++) { Q[i] = V[i] - U[i]; R[i] = V[i] + U[i]; }
     lu(n, Q, piv);
     lu_solve(n, Q, piv, R);
-    double dR[MAXP][MAXN * MAXN];
     for (int p = 0; p < P; p++) {
         double dQ[MAXN * MAXN];
         for (int i = 0; i < nn; i++) dQ[i] = dV[p][i] - dU[p][i];
@@ -291,7 +299,11 @@ static void sim_row(const double* p, int nc, int T, const int* typ, const double
                     const int* slot, double rel_h, int nan_rule, double* out_m, double* out_dm) {
     int nt = 1 + nc, N = 3 * nt + 1, n = N + 1, nn = n * n, st = stride_of(nc);
     double M[MAXN], last[MAXN], dM[MAXP][MAXN], dlast[MAXP][MAXN];
-    double X[MAXN * MAXN], dX[MAXP * MAXN * MAXN], E[MAXN * MAXN], dE[MAXP * MAXN * MAXN];
+    double X[MAXN * MAXN], E[MAXN * MAXN];
+    double* heap = (double*)malloc(sizeof(double) * (size_t)MAXP * MAXN * MAXN * 11);
+    double* dX = heap;
+    double* dE = heap + (size_t)MAXP * MAXN * MAXN;
+    double* ws = heap + (size_t)2 * MAXP * MAXN * MAXN;
     double Xp[MAXN * MAXN], Xm[MAXN * MAXN], q[13 + 5 * 8];
     for (int i = 0; i < N; i++) { M[i] = m0[i]; last[i] = M[i]; }
     for (int k = 0; k < P; k++) for (int i = 0; i < N; i++) { dM[k][i] = dm0 ? dm0[k * N + i] : 0.0; dlast[k][i] = dM[k][i]; }
@@ -309,7 +321,7 @@ static void sim_row(const double* p, int nc, int T, const int* typ, const double
                 for (int i = 0; i < N; i++) M[i] = NAN;
                 for (int k = 0; k < P; k++) for (int i = 0; i < N; i++) dM[k][i] = NAN;
             } else {
-                expm_dual(n, X, P, dX, E, dE);
+                expm_dual(n, X, P, dX, E, dE, ws);
                 double Mn[MAXN], dMn[MAXP][MAXN];
                 for (int i = 0; i < N; i++) {
                     double s = E[i * n + N];
@@ -339,6 +351,7 @@ static void sim_row(const double* p, int nc, int T, const int* typ, const double
     }
     memcpy(out_m, last, sizeof(double) * N);
     for (int k = 0; k < P; k++) memcpy(out_dm + k * N, dlast[k], sizeof(double) * N);
+    free(heap);
 }
 
 /*
@@ -370,8 +383,10 @@ int oracle_simulate(int V, int nc, int O, int T, const double* params, const int
 /* Batched float64 expm with tangents (for checking the oracle itself against scipy). */
 int oracle_expm(int B, int n, const double* X, int P, const double* dX, double* E, double* dE) {
     if (n > MAXN || P > MAXP) return -1;
+    double* ws = (double*)malloc(sizeof(double) * (size_t)MAXP * MAXN * MAXN * 9);
     for (int b = 0; b < B; b++)
         expm_dual(n, X + (long)b * n * n, (dX && dE) ? P : 0, dX ? dX + (long)b * P * n * n : NULL, E + (long)b * n * n,
-                  dE ? dE + (long)b * P * n * n : NULL);
+                  dE ? dE + (long)b * P * n * n : NULL, ws);
+    free(ws);
     return 0;
 }
