@@ -5,7 +5,8 @@ count s (the same 1-norm and theta_m table) and counts the useful float64 operat
   generator apply X v       29 NC + 27 flops (13 NC + 13 FMA, 3 NC + 1 multiplies)
   tangent apply dX v        per slot kind (below), plus X dv
   Taylor term bookkeeping   2 N per vector (scale by 1/k, accumulate)
-  dense product (column)    2 N^2 (1 + 2 PT) per column, N + 1 useful columns
+  dense product (column)    2 N^2 (1 + 2 PT) per column, N + 1 useful columns (squarings, and chaining
+                            an event that needed squarings; other chained events need none)
   apply to the state        (N + 1)(N + 3 PT N) + N (1 + PT) N adds of the team reduction
 Only the N + 1 columns of a propagator count, not the idle lanes of a 16-lane team.
 `reference_flops` is the survey's model of the reference's per-event work (every event its own
@@ -136,7 +137,9 @@ def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1):
                     f = np.where(vec, m * term, NA * m * term + s * mul + app)
                     total += f.sum(axis=1)
                 else:
-                    f = NA * m * term + s * mul + mul
+                    # chained: an event without squarings runs its series on the chained columns
+                    # directly; one with squarings forms its propagator and multiplies it in
+                    f = np.where(s == 0, NA * m * term, NA * m * term + s * mul + mul)
                     total += f.sum(axis=1) + cnt * app
     return total
 

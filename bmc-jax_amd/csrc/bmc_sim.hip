@@ -15,7 +15,9 @@
 //    (s == 0) acts on the state directly with the Taylor series; otherwise its propagator is built
 //    by columns and applied with a team reduction.
 //  * Runs of identical events (a repeated pulse and its gap) are chained into one propagator and
-//    applied `count` times (the row program built on the host).
+//    applied `count` times (the row program built on the host). An event that needs no squaring is
+//    chained by running its Taylor series on each lane's column of the chained propagator, which
+//    needs no dense product at all; only events with squarings form their own propagator first.
 // Tangents travel beside every quantity (dual numbers): d(X v) = dX v + X dv, where dX is the
 // analytic derivative of the generator w.r.t. one packed parameter slot.
 // All arithmetic is float64 (CDNA4 issues FP64 FMA at the FP32 rate).
@@ -35,6 +37,12 @@ __constant__ double kTheta[41] = {0.0,      2.22e-16, 2.581e-08, 1.386e-05, 0.00
                                   1.624,    1.816,    2.015,     2.219,     2.429,     2.643,    2.861,
                                   3.084,    3.31,     3.54,      3.772,     4.008,     4.245,    4.486,
                                   4.728,    4.973,    5.219,     5.468,     5.717,     5.969};
+// 1 / k for the Taylor recursion (a table load instead of a float64 division per term)
+__constant__ double kInv[41] = {0.0, 1.0, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 5, 1.0 / 6, 1.0 / 7, 1.0 / 8, 1.0 / 9,
+                                1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
+                                1.0 / 18, 1.0 / 19, 1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23, 1.0 / 24, 1.0 / 25,
+                                1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29, 1.0 / 30, 1.0 / 31, 1.0 / 32, 1.0 / 33,
+                                1.0 / 34, 1.0 / 35, 1.0 / 36, 1.0 / 37, 1.0 / 38, 1.0 / 39, 1.0 / 40};
 constexpr int kMMax = 30;  // the most accurate cap for the squaring path (fewest squarings before the rounding of the series takes over)
 
 __device__ __forceinline__ double load_real(const void* p, long i, int dtype) {
@@ -122,6 +130,7 @@ __device__ MtShape mt_shape(int ls, double off, double t2, double w0) {
       double px[4] = {-300.0 - w0, -100.0 - w0, 100.0 + w0, 300.0 + w0};
       double dpx[4] = {-1.0, -1.0, 1.0, 1.0};
       double py[4], pyd[4], pyt[4];
+#pragma unroll
       for (int i = 0; i < 4; i++) sl_sum(px[i], t2, py[i], pyd[i], pyt[i]);
       double d0y = 30.0 * (py[1] - py[0]), d1y = 30.0 * (py[3] - py[2]);
       double den = px[2] - px[1] + 1.0;
@@ -137,6 +146,7 @@ __device__ MtShape mt_shape(int ls, double off, double t2, double w0) {
       // w0 moves the four nodes and the interval
       double dq = (-dpx[1] * den - (off - px[1] + 1.0) * (dpx[2] - dpx[1])) / (den * den);
       double pyw[4];
+#pragma unroll
       for (int i = 0; i < 4; i++) pyw[i] = pyd[i] * dpx[i];
       r.g_w0 = g_c * sg * dq + h0 * pyw[1] + h1 * pyw[2] + h2 * 30.0 * (pyw[1] - pyw[0]) +
                h3 * 30.0 * (pyw[3] - pyw[2]);
@@ -491,7 +501,7 @@ __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const 
     }
   double taug = vaug0;
   for (int k = 1; k <= m; k++) {
-    const double ik = 1.0 / (double)k;
+    const double ik = kInv[k];
     double y[N];
     apply_X<NC>(G, t, taug, y);
 #pragma unroll
@@ -582,20 +592,11 @@ __device__ __forceinline__ void event_cols(const Gen<NC>& G, const TanCo* tc, co
                                            const double* dwp, int m, int s, bool is_nan, int j, double* R,
                                            double (*dR)[3 * (1 + NC) + 1]) {
   constexpr int N = 3 * (1 + NC) + 1;
-  double e[N];
 #pragma unroll
-  for (int i = 0; i < N; i++) e[i] = (i == j) ? 1.0 : 0.0;
-  taylor<NC, PT>(G, tc, kind, kidx, dwp, m, e, (j == N) ? 1.0 : 0.0, nullptr, R, dR);
-  for (int q = 0; q < s; q++) {
-    double Rc[N], dRc[PT > 0 ? PT : 1][N];
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      Rc[i] = R[i];
-#pragma unroll
-      for (int p = 0; p < PT; p++) dRc[p][i] = dR[p][i];
-    }
-    mul_cols<NC, PT, TEAM>(Rc, dRc, R, dR, j);
-  }
+  for (int i = 0; i < N; i++) R[i] = (i == j) ? 1.0 : 0.0;
+  taylor<NC, PT>(G, tc, kind, kidx, dwp, m, R, (j == N) ? 1.0 : 0.0, nullptr, R, dR);  // column j of T_m(X)
+  // squarings: mul_cols finishes the product before it writes back, so R may be both factors
+  for (int q = 0; q < s; q++) mul_cols<NC, PT, TEAM>(R, dR, R, dR, j);
   if (is_nan) {
 #pragma unroll
     for (int i = 0; i < N; i++) {
@@ -613,6 +614,9 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
   using I = PrmIdx<NC>;
   constexpr int PTA = PT > 0 ? PT : 1;
   __shared__ double sh_prm[TPB][I::size];
+  __shared__ double sh_state[TPB][(1 + PTA) * N];  // the state, parked while a block is chained
+  __shared__ double sh_start[TPB][(1 + PTA) * N];  // the start state and its tangent seeds
+  __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
 
   const int lane = threadIdx.x % TEAM;
   const int tib = threadIdx.x / TEAM;
@@ -626,6 +630,10 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
   const int q = (int)(t2 / a.V);
   if (!dead && a.active && !a.active[v]) dead = true;
 
+  if (threadIdx.x < kMaxTan) {
+    sh_tk[threadIdx.x] = threadIdx.x < a.P ? a.t_kind[threadIdx.x] : TK_NONE;
+    sh_ti[threadIdx.x] = threadIdx.x < a.P ? a.t_idx[threadIdx.x] : 0;
+  }
   // parameters -> LDS (written slots applied, as write_params does: fitting/params.py:100-112)
   double* prm = sh_prm[tib];
   for (int i = lane; i < I::stride; i += TEAM) {
@@ -636,6 +644,18 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
         val = (a.dtype == BMC_F32) ? (double)(float)xv : xv;
       }
     prm[i] = val;
+  }
+  // the start state and its seeds -> LDS (reloaded at every ADC reset)
+  double* st0 = sh_start[tib];
+  for (int i = lane; i < (1 + PTA) * N; i += TEAM) {
+    const int p = i / N - 1, r = i % N;
+    const int gp = q * PT + p;
+    double val = 0.0;
+    if (p < 0)
+      val = load_real(a.m0, (long)v * N + r, a.dtype);
+    else if (p < PT && gp < a.P && a.dm0)
+      val = load_real(a.dm0, ((long)v * a.P + gp) * N + r, a.dtype);
+    st0[i] = val;
   }
   __syncthreads();
   if (lane == 0) {
@@ -659,32 +679,25 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
 #pragma unroll
   for (int p = 0; p < PTA; p++) {
     const int gp = q * PT + p;
-    kind[p] = (p < PT && gp < a.P) ? a.t_kind[gp] : TK_NONE;
-    kidx[p] = (p < PT && gp < a.P) ? a.t_idx[gp] : 0;
+    kind[p] = (p < PT && gp < a.P) ? sh_tk[gp] : TK_NONE;
+    kidx[p] = (p < PT && gp < a.P) ? sh_ti[gp] : 0;
   }
   double dwp[NC > 0 ? NC : 1];
 #pragma unroll
   for (int i = 0; i < NC; i++) dwp[i] = prm[13 + 2 * NC + i];
 
-  // initial state
-  double M0[N], M[N];
-  double dM0[PTA][N], dM[PTA][N];
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    M0[i] = load_real(a.m0, (long)v * N + i, a.dtype);
-    M[i] = M0[i];
-  }
-#pragma unroll
-  for (int p = 0; p < PTA; p++) {
-    const int gp = q * PT + p;
+  // state (identical on every lane of the team); the start state is reloaded from memory on a reset
+  double M[N];
+  double dM[PTA][N];
+  auto load_start = [&]() {
 #pragma unroll
     for (int i = 0; i < N; i++) {
-      double d = 0.0;
-      if (p < PT && gp < a.P && a.dm0) d = load_real(a.dm0, ((long)v * a.P + gp) * N + i, a.dtype);
-      dM0[p][i] = d;
-      dM[p][i] = d;
+      M[i] = st0[i];
+#pragma unroll
+      for (int p = 0; p < PTA; p++) dM[p][i] = st0[(1 + p) * N + i];
     }
-  }
+  };
+  load_start();
 
   MtCache mc;
   mc.fr = __builtin_nan("");
@@ -696,15 +709,12 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
   for (int oi = op0; oi < op1; oi++) {
     const Op op = a.ops[oi];
     if (op.kind == OP_SEQ) {
+      Ev e_next = evr[op.e0];
       for (int ei = op.e0; ei < op.e0 + op.len; ei++) {
-        const Ev e = evr[ei];
+        const Ev e = e_next;
+        if (ei + 1 < op.e0 + op.len) e_next = evr[ei + 1];
         if (pending) {
-#pragma unroll
-          for (int i = 0; i < N; i++) {
-            M[i] = M0[i];
-#pragma unroll
-            for (int p = 0; p < PTA; p++) dM[p][i] = dM0[p][i];
-          }
+          load_start();
           pending = false;
         }
         if (e.dt > 1e-9) {  // simulate.py:51
@@ -715,13 +725,14 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
           bool isn;
           build_gen<NC, PT>(prm, e, mc, kind, kidx, G, tc, l1, sc, a.nan_l1, m, s, isn);
           if (s == 0) {
-            double Mn[N], dMn[PTA][N];
-            taylor<NC, PT>(G, tc, kind, kidx, dwp, m, M, 1.0, dM, Mn, dMn);
+            taylor<NC, PT>(G, tc, kind, kidx, dwp, m, M, 1.0, dM, M, dM);  // in place
+            if (isn) {
 #pragma unroll
-            for (int i = 0; i < N; i++) {
-              M[i] = isn ? __builtin_nan("") : Mn[i];
+              for (int i = 0; i < N; i++) {
+                M[i] = __builtin_nan("");
 #pragma unroll
-              for (int p = 0; p < PTA; p++) dM[p][i] = isn ? __builtin_nan("") : dMn[p][i];
+                for (int p = 0; p < PTA; p++) dM[p][i] = __builtin_nan("");
+              }
             }
           } else {
             double R[N], dR[PTA][N];
@@ -746,13 +757,18 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
       }
     } else {
       if (pending) {
+        load_start();
+        pending = false;
+      }
+      // the state is not touched while the block is chained: park it in LDS
+      double* st = sh_state[tib];
+      if (lane == 0) {
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          M[i] = M0[i];
+          st[i] = M[i];
 #pragma unroll
-          for (int p = 0; p < PTA; p++) dM[p][i] = dM0[p][i];
+          for (int p = 0; p < PTA; p++) st[(1 + p) * N + i] = dM[p][i];
         }
-        pending = false;
       }
       double Pi[N], dPi[PTA][N];
 #pragma unroll
@@ -761,8 +777,10 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
 #pragma unroll
         for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
       }
+      Ev e_next = evr[op.e0];
       for (int ei = op.e0; ei < op.e0 + op.len; ei++) {
-        const Ev e = evr[ei];
+        const Ev e = e_next;
+        if (ei + 1 < op.e0 + op.len) e_next = evr[ei + 1];  // the next record is in flight meanwhile
         if (e.dt > 1e-9) {
           Gen<NC> G;
           TanCo tc[PTA];
@@ -770,9 +788,23 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
           int m, s;
           bool isn;
           build_gen<NC, PT>(prm, e, mc, kind, kidx, G, tc, l1, sc, a.nan_l1, m, s, isn);
-          double R[N], dR[PTA][N];
-          event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
-          mul_cols<NC, PT, TEAM>(R, dR, Pi, dPi, lane);
+          if (s == 0) {
+            // exp(X) Pi = T_m(X) Pi: each lane runs the series on its own column of the chained
+            // propagator (homogeneous coordinate Pi[N][j] = [j == N]); no dense product is needed
+            taylor<NC, PT>(G, tc, kind, kidx, dwp, m, Pi, (lane == N) ? 1.0 : 0.0, dPi, Pi, dPi);  // in place
+            if (isn) {
+#pragma unroll
+              for (int i = 0; i < N; i++) {
+                Pi[i] = __builtin_nan("");
+#pragma unroll
+                for (int p = 0; p < PTA; p++) dPi[p][i] = __builtin_nan("");
+              }
+            }
+          } else {
+            double R[N], dR[PTA][N];
+            event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
+            mul_cols<NC, PT, TEAM>(R, dR, Pi, dPi, lane);
+          }
         }
         if (e.type == BMC_EV_GRADIENT || e.type == BMC_EV_SPOILER) {
 #pragma unroll
@@ -782,6 +814,12 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
             for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
           }
         }
+      }
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        M[i] = st[i];
+#pragma unroll
+        for (int p = 0; p < PTA; p++) dM[p][i] = st[(1 + p) * N + i];
       }
       for (int c = 0; c < op.count; c++) apply_cols<NC, PT, TEAM>(Pi, dPi, M, dM, lane);
     }

@@ -100,7 +100,7 @@ def test_pulsed_train_every_tangent(nat, lineshape):
         for c in range(len(part)):
             scale = max(np.max(np.abs(J_o[:, :, c])), 1e-12)
             err = np.max(np.abs(J_g[:, :, c] - J_o[:, :, c])) / scale
-            assert err < 1e-6 or np.max(np.abs(J_g[:, :, c] - J_o[:, :, c])) < 1e-12, \
+            assert err < 3e-6 or np.max(np.abs(J_g[:, :, c] - J_o[:, :, c])) < 1e-10, \
                 f"slot {part[c]}: rel {err:.2e}"
     st = plan.stats()
     assert st["n_built"] < ev["dt"].size / 2, "the repeated pulses were not chained"
