@@ -1,0 +1,251 @@
+"""The learned Levenberg-Marquardt proposal: an MLP maps measured Z-spectra to normalised parameters,
+which become the LM start; plain x0 remains the fallback wherever the NN-started fit stays above a
+residual threshold. Not in the reference.
+
+Everything runs on the GPU: the training spectra come from the simulate kernel (bmc_simulate), the
+network is trained and evaluated by libbmc_hip's BF16 MFMA kernels (bmc_mlp_*), and the fits are the
+GPU-resident LM (bmc_lm_fit).
+
+The reference configuration (a 2-pool CW QUESP ladder):
+  pools   water (T1 1.3 s, T2 75 ms) + amide (+3.5 ppm, T1 1.3 s, T2 0.1 s), B0 3 T, no MT
+  ladder  CW saturation 2 s at B1 {0.5, 1, 2, 4} uT, 61 offsets in [-6, 6] ppm -> 244 inputs
+  free    k (log-uniform 10..1000 /s), f (1e-4..5e-3), water R1 (1/2.5..1/0.8), water R2 (1/0.15..1/0.04),
+          B0 shift (-0.3..0.3 ppm) -> 5 outputs, each normalised to [0, 1]
+  net     244 -> 512 -> 512 -> 256 -> 5, GELU, Adam lr 1e-3, batch 4096, sigma 1e-3 noise on Z
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+from .fitting.params import FitSpec, FreeParam
+from .simulation.params import SimParams
+
+LADDER_UT = (0.5, 1.0, 2.0, 4.0)
+PPM = np.linspace(-6.0, 6.0, 61)
+B0_T = 3.0
+DIMS = (244, 512, 512, 256, 5)
+# (field, index, lo, hi, log) of the 5 outputs
+OUTPUTS = (("cest_k", 0, 10.0, 1000.0, True), ("cest_f", 0, 1e-4, 5e-3, False),
+           ("water_r1", None, 1 / 2.5, 1 / 0.8, False), ("water_r2", None, 1 / 0.15, 1 / 0.04, False),
+           ("b0_inhom", None, -0.3, 0.3, False))
+
+
+def ladder_events():
+    from .sequences import cw_saturation
+    from .simulation.events import concat_events
+    return concat_events([cw_saturation(PPM, b1_uT=b, tsat_s=2.0, b0_T=B0_T) for b in LADDER_UT])
+
+
+def fit_spec():
+    """The LM's free values for this configuration (scales make every value O(1))."""
+    return FitSpec([FreeParam("k", "cest_k", 0, 100.0, 5.0, 5000.0, 100.0),
+                    FreeParam("f", "cest_f", 0, 0.001, 1e-5, 0.02, 0.001),
+                    FreeParam("r1w", "water_r1", None, 1.0, 0.2, 2.0, 1 / 1.3),
+                    FreeParam("r2w", "water_r2", None, 10.0, 2.0, 50.0, 1 / 0.075),
+                    FreeParam("db0", "b0_inhom", None, 0.1, -0.5, 0.5, 0.0)])
+
+
+def draw_params(n, rng):
+    """(physical values [n, 5], normalised [n, 5]) uniform / log-uniform over OUTPUTS."""
+    u = rng.uniform(0.0, 1.0, (n, len(OUTPUTS)))
+    return denormalise(u), u
+
+
+def denormalise(u):
+    u = np.asarray(u, np.float64)
+    out = np.empty_like(u)
+    for j, (_, _, lo, hi, lg) in enumerate(OUTPUTS):
+        out[:, j] = np.exp(np.log(lo) + u[:, j] * (np.log(hi) - np.log(lo))) if lg else lo + u[:, j] * (hi - lo)
+    return out
+
+
+def normalise(x):
+    x = np.asarray(x, np.float64)
+    out = np.empty_like(x)
+    for j, (_, _, lo, hi, lg) in enumerate(OUTPUTS):
+        out[:, j] = (np.log(x[:, j]) - np.log(lo)) / (np.log(hi) - np.log(lo)) if lg else (x[:, j] - lo) / (hi - lo)
+    return out
+
+
+def base_params(n, dtype=np.float32):
+    """n copies of the configuration's pool model, as a batched SimParams."""
+    f = lambda v: np.full(n, v, dtype)
+    c = lambda v: np.full((n, 1), v, dtype)
+    return SimParams(water_r1=f(1 / 1.3), water_r2=f(1 / 0.075), water_f=f(1.0), cest_k=c(50.0), cest_f=c(9.01e-4),
+                     cest_dw=c(3.5), cest_r1=c(1 / 1.3), cest_r2=c(10.0), mt_k=f(0), mt_f=f(0), mt_dw=f(0), mt_r1=f(0),
+                     mt_r2=f(1.0), mt_lineshape=np.zeros(n, np.int32), b0=f(B0_T), gamma=f(267.5153),
+                     b0_inhom=f(0.0), rel_b1=f(1.0))
+
+
+def with_values(base: SimParams, x) -> SimParams:
+    upd = {}
+    dt = np.asarray(base.water_r1).dtype
+    for j, (field, idx, *_rest) in enumerate(OUTPUTS):
+        if idx is None:
+            upd[field] = np.asarray(x[:, j], dt)
+        else:
+            cur = np.array(upd.get(field, getattr(base, field)), copy=True)
+            cur[:, idx] = x[:, j]
+            upd[field] = cur
+    return base._replace(**upd)
+
+
+def simulate_ladder(x, chunk=65536, device_out=False):
+    """Z [n, 244] of the ladder for physical values x [n, 5] (GPU, float32)."""
+    from . import _engine
+    from .simulation.params import initial_state
+    torch = _native.torch_cuda()
+    ev = ladder_events()
+    n = x.shape[0]
+    outs = []
+    for s in range(0, n, chunk):
+        xs = x[s:s + chunk]
+        p = with_values(base_params(xs.shape[0]), xs)
+        prm, m0, nc, V, code, npd = _engine.prepare(p, initial_state(p), n_voxels=xs.shape[0], dtype=np.float32)
+        plan = _engine.plan_for(ev)
+        m, _ = _native.simulate(plan, prm, nc, code, m0, want_jac=False)
+        outs.append(m[..., 4].contiguous())
+    z = torch.cat(outs, 0)
+    return z if device_out else z.cpu().numpy()
+
+
+class MLP:
+    """Handle of a device MLP (bmc_mlp_*)."""
+
+    def __init__(self, dims=DIMS, max_batch=4096, seed=5):
+        L = _native.lib()
+        _bind(L)
+        self.dims = tuple(int(d) for d in dims)
+        arr = (ctypes.c_int32 * len(self.dims))(*self.dims)
+        h = ctypes.c_void_p()
+        _native.check(L.bmc_mlp_create(len(self.dims) - 1, arr, int(max_batch), ctypes.c_uint64(seed), ctypes.byref(h)),
+                      "bmc_mlp_create")
+        self._h = h
+        self.max_batch = int(max_batch)
+        self.n_params = int(L.bmc_mlp_n_params(h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _native._lib is not None:
+            _native._lib.bmc_mlp_destroy(self._h)
+            self._h = None
+
+    def get_params(self):
+        out = np.zeros(self.n_params, np.float32)
+        _native.check(_native.lib().bmc_mlp_get_params(self._h, out.ctypes.data_as(ctypes.c_void_p)),
+                      "bmc_mlp_get_params")
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _native.check(_native.lib().bmc_mlp_set_params(self._h, w.ctypes.data_as(ctypes.c_void_p)),
+                      "bmc_mlp_set_params")
+
+    def forward(self, x):
+        """x: device float32 tensor [n, dims[0]] -> device tensor [n, dims[-1]] (chunked by max_batch)."""
+        torch = _native.torch_cuda()
+        x = x.contiguous()
+        n = x.shape[0]
+        y = torch.empty((n, self.dims[-1]), dtype=torch.float32, device="cuda")
+        for s in range(0, n, self.max_batch):
+            b = min(self.max_batch, n - s)
+            _native.check(_native.lib().bmc_mlp_forward(self._h, ctypes.c_void_p(x[s:s + b].data_ptr()), b,
+                                                        ctypes.c_void_p(y[s:s + b].data_ptr()), _native.stream_ptr()),
+                          "bmc_mlp_forward")
+        return y
+
+    def gradients(self, x, t):
+        """(flat FP32 gradient of the mean-squared error, loss) for one batch (device tensors)."""
+        g = np.zeros(self.n_params, np.float32)
+        loss = ctypes.c_float()
+        _native.check(_native.lib().bmc_mlp_gradients(self._h, ctypes.c_void_p(x.data_ptr()),
+                                                      ctypes.c_void_p(t.data_ptr()), int(x.shape[0]),
+                                                      g.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss),
+                                                      _native.stream_ptr()), "bmc_mlp_gradients")
+        return g, float(loss.value)
+
+    def train(self, X, T, epochs=10, batch=4096, lr=1e-3, seed=0, log=None):
+        """Adam on mini-batches of device tensors X [n, in], T [n, out]. Returns the mean loss per epoch."""
+        torch = _native.torch_cuda()
+        n = X.shape[0]
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed)
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        hist = []
+        L = _native.lib()
+        for ep in range(epochs):
+            perm = torch.randperm(n, device="cuda", generator=gen)
+            loss.zero_()
+            steps = 0
+            for s in range(0, n - batch + 1, batch):
+                idx = perm[s:s + batch]
+                xb, tb = X[idx].contiguous(), T[idx].contiguous()
+                _native.check(L.bmc_mlp_train_step(self._h, ctypes.c_void_p(xb.data_ptr()), ctypes.c_void_p(tb.data_ptr()),
+                                                   batch, ctypes.c_float(lr), ctypes.c_void_p(loss.data_ptr()),
+                                                   _native.stream_ptr()), "bmc_mlp_train_step")
+                steps += 1
+            hist.append(float(loss.item()) / max(steps, 1))
+            if log:
+                log(f"epoch {ep}: loss {hist[-1]:.3e}")
+        return hist
+
+
+_BOUND = False
+
+
+def _bind(L):
+    global _BOUND
+    if _BOUND:
+        return
+    vp = ctypes.c_void_p
+    L.bmc_mlp_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int32), ctypes.c_int, ctypes.c_uint64,
+                                 ctypes.POINTER(vp)]
+    L.bmc_mlp_destroy.argtypes = [vp]
+    L.bmc_mlp_n_params.argtypes = [vp]
+    L.bmc_mlp_n_params.restype = ctypes.c_int64
+    L.bmc_mlp_set_params.argtypes = [vp, vp]
+    L.bmc_mlp_get_params.argtypes = [vp, vp]
+    L.bmc_mlp_forward.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+    L.bmc_mlp_gradients.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_float), vp]
+    L.bmc_mlp_train_step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_float, vp, vp]
+    for n in ("bmc_mlp_create", "bmc_mlp_destroy", "bmc_mlp_set_params", "bmc_mlp_get_params", "bmc_mlp_forward",
+              "bmc_mlp_gradients", "bmc_mlp_train_step"):
+        getattr(L, n).restype = ctypes.c_int
+    _BOUND = True
+
+
+@dataclass
+class Proposal:
+    """A trained proposal network and what it was trained on."""
+    mlp: MLP
+    loss: list
+    train_seconds: float
+
+
+def train_proposal(n_train=2_000_000, epochs=10, batch=4096, lr=1e-3, noise=1e-3, seed=5, log=None) -> Proposal:
+    """Simulate n_train spectra on the GPU, add sigma-noise, train the MLP (BF16 MFMA) to the normalised values."""
+    torch = _native.torch_cuda()
+    rng = np.random.default_rng(seed)
+    x, u = draw_params(n_train, rng)
+    t0 = time.perf_counter()
+    Z = simulate_ladder(x, device_out=True)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    Z = Z + noise * torch.randn(Z.shape, generator=gen, device="cuda", dtype=torch.float32)
+    T = torch.as_tensor(u.astype(np.float32), device="cuda")
+    mlp = MLP(max_batch=batch, seed=seed)
+    hist = mlp.train(Z, T, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
+    torch.cuda.synchronize()
+    return Proposal(mlp, hist, time.perf_counter() - t0)
+
+
+def propose(prop: Proposal, Z):
+    """Physical start values [n, 5] for measured spectra Z [n, 244] (device tensor or NumPy)."""
+    torch = _native.torch_cuda()
+    z = Z if hasattr(Z, "data_ptr") else torch.as_tensor(np.asarray(Z, np.float32), device="cuda")
+    u = prop.mlp.forward(z.to(torch.float32)).cpu().numpy()
+    return denormalise(np.clip(u, 0.0, 1.0))

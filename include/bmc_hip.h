@@ -144,6 +144,26 @@ int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const 
                double lam0, const double* free_dev, unsigned flags, double early_ftol, double early_rms,
                double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev, void* stream);
 
+/*
+ * The learned LM proposal: an MLP trained and evaluated with hand-written BF16 MFMA GEMMs (FP32
+ * accumulate), GELU hidden layers, a linear output, mean-squared-error loss and Adam. No counterpart
+ * in the reference; its output is the LM start x0 (plain x0 remains the fallback).
+ *   dims [n_layers + 1] layer widths; weights W_l [dims[l+1], dims[l]] then b_l, flat FP32 parameters.
+ *   x_dev [batch, dims[0]] FP32, t_dev / y_dev [batch, dims[n_layers]] FP32 (device).
+ * bmc_mlp_train_step adds the batch loss into *loss_dev (device float; the caller clears it).
+ */
+typedef struct bmc_mlp bmc_mlp;
+int bmc_mlp_create(int n_layers, const int32_t* dims, int max_batch, uint64_t seed, bmc_mlp** out);
+int bmc_mlp_destroy(bmc_mlp* mlp);
+int64_t bmc_mlp_n_params(const bmc_mlp* mlp);
+int bmc_mlp_set_params(bmc_mlp* mlp, const float* host_params);
+int bmc_mlp_get_params(const bmc_mlp* mlp, float* host_params);
+int bmc_mlp_forward(bmc_mlp* mlp, const float* x_dev, int batch, float* y_dev, void* stream);
+int bmc_mlp_gradients(bmc_mlp* mlp, const float* x_dev, const float* t_dev, int batch, float* grad_host,
+                      float* loss_host, void* stream);
+int bmc_mlp_train_step(bmc_mlp* mlp, const float* x_dev, const float* t_dev, int batch, float lr, float* loss_dev,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif
