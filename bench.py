@@ -92,6 +92,11 @@ def main():
     ap.add_argument("--voxels", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flop-sample", type=int, default=16)
+    ap.add_argument("--workload", choices=("pulsed", "nn"), default="pulsed",
+                    help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
+    ap.add_argument("--nn-train", type=int, default=2_000_000)
+    ap.add_argument("--nn-test", type=int, default=262_144)
+    ap.add_argument("--nn-epochs", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (voxels are independent: "
@@ -101,6 +106,23 @@ def main():
     import torch
     from bmc_jax import _engine, _flops, _native
     from bmc_jax.simulation.events import SimEvent
+
+    if args.workload == "nn":
+        from bmc_jax import learned
+        _native.torch_cuda()
+        r = learned.benchmark(n_train=args.nn_train, n_test=args.nn_test, epochs=args.nn_epochs,
+                              log=lambda m: print(m, file=sys.stderr, flush=True))
+        out = {"metric": "fitted voxels/s (NN-initialised LM with plain fallback, rms <= 1e-4, noise-free)",
+               "value": r["nn"]["fitted_per_s"], "unit": "fitted voxels/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+               "ms_per_step": 1e3 * r["nn"]["seconds"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f64 simulate / bf16-fp32 MLP",
+               "data": "synthetic (simulator-generated, seeded)",
+               "config": {"workload": "learned LM proposal: 2-pool CW QUESP ladder (0.5, 1, 2, 4 uT, 2 s), 61 offsets, "
+                                      "MLP 244-512-512-256-5 GELU", "n_train": args.nn_train, "n_test": args.nn_test},
+               "plain_lm": r["plain"], "nn_lm": r["nn"], "speedup_vs_plain": r["speedup"],
+               "train_seconds": r["train_seconds"]}
+        print(json.dumps(out), flush=True)
+        return
 
     _native.torch_cuda()
     prm, m0, ev, nc = workload(args.voxels)

@@ -96,7 +96,11 @@ def with_values(base: SimParams, x) -> SimParams:
 
 
 def simulate_ladder(x, chunk=65536, device_out=False):
-    """Z [n, 244] of the ladder for physical values x [n, 5] (GPU, float32)."""
+    """Z [n, 244] of the ladder for physical values x [n, 5] (GPU, float32).
+
+    Every spectrum starts from the configuration's base start state (initial_state(base_params)), the
+    M0 that the fits hold fixed whatever f they try (fit_batch takes M0 per voxel and never rebuilds it:
+    src/bmc_jax/fitting/independent.py:82-98), so that the fit model is exact for these spectra."""
     from . import _engine
     from .simulation.params import initial_state
     torch = _native.torch_cuda()
@@ -105,8 +109,9 @@ def simulate_ladder(x, chunk=65536, device_out=False):
     outs = []
     for s in range(0, n, chunk):
         xs = x[s:s + chunk]
-        p = with_values(base_params(xs.shape[0]), xs)
-        prm, m0, nc, V, code, npd = _engine.prepare(p, initial_state(p), n_voxels=xs.shape[0], dtype=np.float32)
+        base = base_params(xs.shape[0])
+        p = with_values(base, xs)
+        prm, m0, nc, V, code, npd = _engine.prepare(p, initial_state(base), n_voxels=xs.shape[0], dtype=np.float32)
         plan = _engine.plan_for(ev)
         m, _ = _native.simulate(plan, prm, nc, code, m0, want_jac=False)
         outs.append(m[..., 4].contiguous())
@@ -168,8 +173,9 @@ class MLP:
                                                       _native.stream_ptr()), "bmc_mlp_gradients")
         return g, float(loss.value)
 
-    def train(self, X, T, epochs=10, batch=4096, lr=1e-3, seed=0, log=None):
-        """Adam on mini-batches of device tensors X [n, in], T [n, out]. Returns the mean loss per epoch."""
+    def train(self, X, T, epochs=10, batch=4096, lr=1e-3, seed=0, log=None, lr_final=0.05):
+        """Adam on mini-batches of device tensors X [n, in], T [n, out], the step size decaying on a cosine
+        from lr to lr * lr_final over the run. Returns the mean loss per epoch."""
         torch = _native.torch_cuda()
         n = X.shape[0]
         gen = torch.Generator(device="cuda")
@@ -177,6 +183,9 @@ class MLP:
         loss = torch.zeros(1, dtype=torch.float32, device="cuda")
         hist = []
         L = _native.lib()
+        per_epoch = max(n // batch, 1)
+        total = epochs * per_epoch
+        k = 0
         for ep in range(epochs):
             perm = torch.randperm(n, device="cuda", generator=gen)
             loss.zero_()
@@ -184,10 +193,12 @@ class MLP:
             for s in range(0, n - batch + 1, batch):
                 idx = perm[s:s + batch]
                 xb, tb = X[idx].contiguous(), T[idx].contiguous()
+                lr_k = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + np.cos(np.pi * k / total)))
                 _native.check(L.bmc_mlp_train_step(self._h, ctypes.c_void_p(xb.data_ptr()), ctypes.c_void_p(tb.data_ptr()),
-                                                   batch, ctypes.c_float(lr), ctypes.c_void_p(loss.data_ptr()),
+                                                   batch, ctypes.c_float(lr_k), ctypes.c_void_p(loss.data_ptr()),
                                                    _native.stream_ptr()), "bmc_mlp_train_step")
                 steps += 1
+                k += 1
             hist.append(float(loss.item()) / max(steps, 1))
             if log:
                 log(f"epoch {ep}: loss {hist[-1]:.3e}")
@@ -249,3 +260,56 @@ def propose(prop: Proposal, Z):
     z = Z if hasattr(Z, "data_ptr") else torch.as_tensor(np.asarray(Z, np.float32), device="cuda")
     u = prop.mlp.forward(z.to(torch.float32)).cpu().numpy()
     return denormalise(np.clip(u, 0.0, 1.0))
+
+
+def fit_ladder(Z, x0, n_iter=30, rms_tol=1e-4, early_exit=True):
+    """GPU LM fits of the ladder spectra Z [n, 244] from x0 [n, 5]; returns (x, rms, iterations run)."""
+    from .fitting.independent import _lm_batched
+    from .simulation.params import initial_state
+    spec = fit_spec()
+    n = Z.shape[0]
+    base = base_params(n)
+    x, _, _, nd = _lm_batched(base, ladder_events(), initial_state(base), 4, spec, np.asarray(Z, np.float32),
+                              spec.to_scaled(np.asarray(x0, np.float64)), n_iter, np.float32, early_exit=early_exit,
+                              early_ftol=1e-9, early_rms=rms_tol)
+    X = x * spec.scales
+    zf = simulate_ladder(X)
+    rms = np.sqrt(np.mean((zf - np.asarray(Z, np.float64)) ** 2, axis=1))
+    return X, rms, nd
+
+
+def benchmark(n_train=2_000_000, n_test=262_144, epochs=10, n_iter=30, rms_tol=1e-4, log=None):
+    """Fitted voxels per second, NN-initialised LM (with plain-x0 fallback) vs plain-x0 LM, on noise-free
+    held-out ladders, at equal accuracy (a voxel counts as fitted when its rms <= rms_tol)."""
+    torch = _native.torch_cuda()
+    prop = train_proposal(n_train=n_train, epochs=epochs, log=log)
+    rng = np.random.default_rng(55)
+    x_true, _ = draw_params(n_test, rng)
+    Z = simulate_ladder(x_true)
+    spec = fit_spec()
+    torch.cuda.synchronize()
+    # plain start
+    t0 = time.perf_counter()
+    Xp, rms_p, nd_p = fit_ladder(Z, np.broadcast_to(spec.x0_phys, (n_test, 5)), n_iter, rms_tol)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    # network start, plain start as the fallback where the network's fit stays above the threshold
+    t0 = time.perf_counter()
+    x_nn = propose(prop, Z)
+    Xn, rms_n, nd_n = fit_ladder(Z, x_nn, n_iter, rms_tol)
+    bad = np.flatnonzero(~(rms_n <= rms_tol))
+    if bad.size:
+        Xb, rb, ndb = fit_ladder(Z[bad], np.broadcast_to(spec.x0_phys, (bad.size, 5)), n_iter, rms_tol)
+        better = rb < rms_n[bad]
+        Xn[bad[better]], rms_n[bad[better]] = Xb[better], rb[better]
+    torch.cuda.synchronize()
+    t_nn = time.perf_counter() - t0
+    ok_p, ok_n = rms_p <= rms_tol, rms_n <= rms_tol
+    rel = lambda X: np.median(np.abs(X / x_true - 1.0)[:, :2], axis=0)
+    return dict(train_seconds=prop.train_seconds, final_train_loss=prop.loss[-1], n_test=n_test,
+                plain=dict(seconds=t_plain, fitted=int(ok_p.sum()), fitted_per_s=ok_p.sum() / t_plain,
+                           mean_iterations=float(np.mean(nd_p)), median_rel_err_k_f=rel(Xp).tolist()),
+                nn=dict(seconds=t_nn, fitted=int(ok_n.sum()), fitted_per_s=ok_n.sum() / t_nn,
+                        mean_iterations=float(np.mean(nd_n)), fallback=int(bad.size),
+                        median_rel_err_k_f=rel(Xn).tolist()),
+                speedup=(ok_n.sum() / t_nn) / max(ok_p.sum() / t_plain, 1e-30))

@@ -76,6 +76,7 @@ struct SimArgs {
   const int32_t* active;  // [V] or null: skip voxels whose flag is 0
   double nan_l1;          // reference NaN threshold on the augmented 1-norm
   void* traj;             // [V, O, T, N] dtype or null: state after every event (SEQ-only plans)
+  long team0;             // first team of this launch (a launch covers at most 2^31 threads)
 };
 
 struct Plan {

@@ -22,6 +22,7 @@
 // analytic derivative of the generator w.r.t. one packed parameter slot.
 // All arithmetic is float64 (CDNA4 issues FP64 FMA at the FP32 rate).
 
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -620,7 +621,7 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
 
   const int lane = threadIdx.x % TEAM;
   const int tib = threadIdx.x / TEAM;
-  const long team = (long)blockIdx.x * TPB + tib;
+  const long team = a.team0 + (long)blockIdx.x * TPB + tib;
   const long total = (long)a.V * a.O * a.n_pass;
   bool dead = team >= total;
   const long tt = dead ? 0 : team;
@@ -853,14 +854,15 @@ static int launch_nc_pt(const SimArgs& a, hipStream_t stream) {
   constexpr int TEAM = NA <= 16 ? 16 : 32;
   constexpr int TPB = 256 / TEAM;
   const long teams = (long)a.V * a.O * a.n_pass;
-  const long blocks = (teams + TPB - 1) / TPB;
-  if (blocks <= 0) return BMC_OK;
-  if (blocks > 0x7fffffffL) {
-    set_error("simulate: grid too large");
-    return BMC_EINVAL;
+  // HIP caps a launch at 2^32 - 1 work-items: split the teams into launches of at most 2^23 blocks
+  constexpr long kMaxBlocks = 1L << 23;
+  for (long t0 = 0; t0 < teams; t0 += kMaxBlocks * TPB) {
+    const long blocks = std::min(kMaxBlocks, (teams - t0 + TPB - 1) / TPB);
+    SimArgs c = a;
+    c.team0 = t0;
+    hipLaunchKernelGGL((sim_kernel<NC, PT, TEAM>), dim3((unsigned)blocks), dim3(256), 0, stream, c);
+    BMC_HIP_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL((sim_kernel<NC, PT, TEAM>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
-  BMC_HIP_CHECK(hipGetLastError());
   return BMC_OK;
 }
 
