@@ -72,9 +72,12 @@ def lib():
         L.bmc_lm_fit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ip32, dp, dp, dp,
                                  ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_uint,
                                  ctypes.c_double, ctypes.c_double, vp, vp, vp, vp]
+        L.bmc_adjoint_workspace_size.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, i64p]
+        L.bmc_simulate_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32,
+                                           dp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int64, vp]
         for name in ("bmc_plan_create", "bmc_plan_create_ex", "bmc_plan_destroy", "bmc_plan_stats", "bmc_plan_program",
-                     "bmc_simulate",
-                     "bmc_simulate_trajectory", "bmc_lm_fit"):
+                     "bmc_simulate", "bmc_simulate_trajectory", "bmc_lm_fit", "bmc_adjoint_workspace_size",
+                     "bmc_simulate_adjoint"):
             getattr(L, name).restype = ctypes.c_int
         if L.bmc_abi_version() != 1:
             raise NativeError("libbmc_hip.so ABI version mismatch")
@@ -216,3 +219,28 @@ def lm_fit(plan: Plan, params, n_cest: int, dtype_code: int, m0, slots, scales, 
                            jc, ptr(z_meas), ptr(x), int(n_iter), float(lam0), ptr(fr), int(flags), float(early_ftol),
                            float(early_rms), ptr(ctraj), ptr(xtraj), ptr(ndone), stream_ptr()), "bmc_lm_fit")
     return x, ctraj, xtraj, ndone
+
+
+def simulate_adjoint(plan: Plan, params, n_cest: int, dtype_code: int, m0, slots=(), scales=(), jc=0, z_cot=None,
+                     jac_cot=None, want_amp=True, want_freq=True, dm0=None):
+    """bmc_simulate_adjoint: (dL/d rf_amp [O, T], dL/d rf_freq [O, T]) float64 device tensors (None where not
+    wanted) for cotangents z_cot [V, O] on Z and jac_cot [V, O, P] on the Jacobian of simulate()."""
+    torch = torch_cuda()
+    L = lib()
+    V = int(params.shape[0])
+    P = len(slots)
+    nbytes = ctypes.c_int64()
+    check(L.bmc_adjoint_workspace_size(plan.handle, V, n_cest, P, ctypes.byref(nbytes)), "bmc_adjoint_workspace_size")
+    ws = torch.empty(max(int(nbytes.value), 8), dtype=torch.uint8, device="cuda")
+    shape = (plan.n_offsets, plan.n_events)
+    ga = torch.zeros(shape, dtype=torch.float64, device="cuda") if want_amp else None
+    gf = torch.zeros(shape, dtype=torch.float64, device="cuda") if want_freq else None
+    f64 = lambda t: None if t is None else t.to(device="cuda", dtype=torch.float64).contiguous()
+    zc, jcot = f64(z_cot), f64(jac_cot)
+    sl = (ctypes.c_int32 * max(P, 1))(*[int(s) for s in slots])
+    sc = (ctypes.c_double * max(P, 1))(*[float(s) for s in scales])
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    check(L.bmc_simulate_adjoint(plan.handle, V, n_cest, dtype_code, ptr(params), ptr(m0), ptr(dm0), P, sl, sc, jc,
+                                 ptr(zc), ptr(jcot), ptr(ga), ptr(gf), ptr(ws), int(nbytes.value), stream_ptr()),
+          "bmc_simulate_adjoint")
+    return ga, gf

@@ -1,0 +1,591 @@
+// Device building blocks of the simulate and adjoint kernels: the sparse generator of one event,
+// its tangents, the truncated Taylor series and the column-layout products of a 16-lane team.
+// Included by each HIP translation unit (no relocatable device code: every unit keeps its own copy).
+#pragma once
+
+#include <type_traits>
+#include <utility>
+
+#include "bmc_internal.h"
+
+namespace bmc {
+
+// Taylor degree thresholds theta_m: the largest 1-norm for which the truncated series T_m has
+// backward error <= 2^-53 (Al-Mohy & Higham 2011, Sec. 3; recomputed for this build).
+static __constant__ double kTheta[41] = {0.0,      2.22e-16, 2.581e-08, 1.386e-05, 0.0003397, 0.002401, 0.009066,
+                                  0.02384,  0.04991,  0.08958,   0.1442,    0.2142,    0.2996,   0.3998,
+                                  0.5139,   0.6411,   0.7803,    0.9305,    1.091,     1.26,     1.438,
+                                  1.624,    1.816,    2.015,     2.219,     2.429,     2.643,    2.861,
+                                  3.084,    3.31,     3.54,      3.772,     4.008,     4.245,    4.486,
+                                  4.728,    4.973,    5.219,     5.468,     5.717,     5.969};
+// 1 / k for the Taylor recursion (a table load instead of a float64 division per term)
+static __constant__ double kInv[41] = {0.0, 1.0, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 5, 1.0 / 6, 1.0 / 7, 1.0 / 8, 1.0 / 9,
+                                1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17,
+                                1.0 / 18, 1.0 / 19, 1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23, 1.0 / 24, 1.0 / 25,
+                                1.0 / 26, 1.0 / 27, 1.0 / 28, 1.0 / 29, 1.0 / 30, 1.0 / 31, 1.0 / 32, 1.0 / 33,
+                                1.0 / 34, 1.0 / 35, 1.0 / 36, 1.0 / 37, 1.0 / 38, 1.0 / 39, 1.0 / 40};
+constexpr int kMMax = 30;  // the most accurate cap for the squaring path (fewest squarings before the rounding of the series takes over)
+
+__device__ __forceinline__ double load_real(const void* p, long i, int dtype) {
+  return dtype == BMC_F64 ? static_cast<const double*>(p)[i] : (double)static_cast<const float*>(p)[i];
+}
+__device__ __forceinline__ void store_real(void* p, long i, double v, int dtype) {
+  if (dtype == BMC_F64)
+    static_cast<double*>(p)[i] = v;
+  else
+    static_cast<float*>(p)[i] = (float)v;
+}
+
+template <int K, int END, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (K < END) {
+    f(std::integral_constant<int, K>{});
+    static_for<K + 1, END>(f);
+  }
+}
+
+// Broadcast lane K of the team to every lane of the team.
+template <int TEAM, int K>
+__device__ __forceinline__ double bcast(double x) {
+  if constexpr (TEAM == 16) {
+    return __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);  // row_newbcast:K
+  } else {
+    return __shfl(x, K, TEAM);
+  }
+}
+
+template <int TEAM>
+__device__ __forceinline__ double team_sum(double x) {
+#pragma unroll
+  for (int m = 1; m < TEAM; m <<= 1) x += __shfl_xor(x, m, TEAM);
+  return x;
+}
+
+// ---- parameters --------------------------------------------------------------------------------
+// LDS copy of one voxel's packed record, plus derived values.
+template <int NC>
+struct PrmIdx {
+  static constexpr int stride = 13 + 5 * NC;
+  static constexpr int L = stride, KS = stride + 1, KWMT = stride + 2, DW0 = stride + 3;
+  static constexpr int size = stride + 4;
+};
+
+// MT saturation lineshape g(off, t2, w0) and its partial derivatives (matrix.py:24-81, 183-213).
+struct MtShape {
+  double g, g_off, g_t2, g_w0;
+};
+
+static __device__ void sl_sum(double d, double t2, double& g, double& g_d, double& g_t2) {
+  // matrix.py:34-38: sum_u sqrt(2/pi) t2 / T exp(-2 (d t2 / T)^2) * pi * du, T = max(|3u^2-1|, 1e-6)
+  double s = 0.0, sd = 0.0, st = 0.0;
+  const double c = 1.2533141373155001;  // sqrt(2/pi) * pi * 0.01 * 100 / 100 -> folded below
+  for (int i = 0; i <= 100; i++) {
+    double u = i * 0.01;
+    double T = fabs(3.0 * u * u - 1.0);
+    T = T < 1e-6 ? 1e-6 : T;
+    double a = d * t2 / T;
+    double e = exp(-2.0 * a * a);
+    double base = t2 / T * e;
+    s += base;
+    sd += base * (-4.0 * a * t2 / T);
+    st += e / T * (1.0 - 4.0 * a * a);
+  }
+  const double k = 0.7978845608028654 * 3.141592653589793 * 0.01;  // sqrt(2/pi) * pi * du
+  (void)c;
+  g = s * k;
+  g_d = sd * k;
+  g_t2 = st * k;
+}
+
+static __device__ MtShape mt_shape(int ls, double off, double t2, double w0) {
+  MtShape r{0.0, 0.0, 0.0, 0.0};
+  if (ls == 1) {  // Lorentzian t2 / (1 + (off t2)^2), matrix.py:70-81
+    double q = off * t2, den = 1.0 + q * q;
+    r.g = t2 / den;
+    r.g_off = -2.0 * off * t2 * t2 * t2 / (den * den);
+    r.g_t2 = (1.0 - q * q) / (den * den);
+  } else if (ls == 2) {
+    if (fabs(off) >= w0) {
+      sl_sum(off, t2, r.g, r.g_off, r.g_t2);
+    } else {  // cubic Hermite bridge, matrix.py:40-60
+      double px[4] = {-300.0 - w0, -100.0 - w0, 100.0 + w0, 300.0 + w0};
+      double dpx[4] = {-1.0, -1.0, 1.0, 1.0};
+      double py[4], pyd[4], pyt[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) sl_sum(px[i], t2, py[i], pyd[i], pyt[i]);
+      double d0y = 30.0 * (py[1] - py[0]), d1y = 30.0 * (py[3] - py[2]);
+      double den = px[2] - px[1] + 1.0;
+      double q = (off - px[1] + 1.0) / den;
+      double sg = q > 0.0 ? 1.0 : (q < 0.0 ? -1.0 : 0.0);
+      double c = fabs(q), c2 = c * c, c3 = c2 * c;
+      double h0 = 2 * c3 - 3 * c2 + 1, h1 = -2 * c3 + 3 * c2, h2 = c3 - 2 * c2 + c, h3 = c3 - c2;
+      double h0c = 6 * c2 - 6 * c, h1c = -6 * c2 + 6 * c, h2c = 3 * c2 - 4 * c + 1, h3c = 3 * c2 - 2 * c;
+      r.g = h0 * py[1] + h1 * py[2] + h2 * d0y + h3 * d1y;
+      double g_c = h0c * py[1] + h1c * py[2] + h2c * d0y + h3c * d1y;
+      r.g_off = g_c * sg / den;
+      r.g_t2 = h0 * pyt[1] + h1 * pyt[2] + h2 * 30.0 * (pyt[1] - pyt[0]) + h3 * 30.0 * (pyt[3] - pyt[2]);
+      // w0 moves the four nodes and the interval
+      double dq = (-dpx[1] * den - (off - px[1] + 1.0) * (dpx[2] - dpx[1])) / (den * den);
+      double pyw[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) pyw[i] = pyd[i] * dpx[i];
+      r.g_w0 = g_c * sg * dq + h0 * pyw[1] + h1 * pyw[2] + h2 * 30.0 * (pyw[1] - pyw[0]) +
+               h3 * 30.0 * (pyw[3] - pyw[2]);
+    }
+  }
+  return r;
+}
+
+// ---- the generator -----------------------------------------------------------------------------
+// Entries of X = aug * dt * 2^-s (matrix.py:238-293), the non-zero structure made explicit.
+template <int NC>
+struct Gen {
+  double axx, offw, w1x, w1y, azz, cw;
+  double b[NC > 0 ? NC : 1], o[NC > 0 ? NC : 1], c[NC > 0 ? NC : 1], k[NC > 0 ? NC : 1],
+      kw[NC > 0 ? NC : 1], cc[NC > 0 ? NC : 1];
+  double dmt, kwmt, kmt, cmt;
+};
+
+// y = X v (v augmented with vaug, the homogeneous coordinate).
+template <int NC>
+__device__ __forceinline__ void apply_X(const Gen<NC>& G, const double* v, double vaug, double* y) {
+  constexpr int NT = 1 + NC;
+  constexpr int xw = 0, yw = NT, zw = 2 * NT, zmt = 3 * NT;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+  for (int i = 0; i < NC; i++) {
+    sx = fma(G.k[i], v[1 + i], sx);
+    sy = fma(G.k[i], v[NT + 1 + i], sy);
+    sz = fma(G.k[i], v[2 * NT + 1 + i], sz);
+  }
+  y[xw] = fma(G.axx, v[xw], fma(G.offw, v[yw], fma(-G.w1y, v[zw], sx)));
+  y[yw] = fma(-G.offw, v[xw], fma(G.axx, v[yw], fma(G.w1x, v[zw], sy)));
+  y[zw] = fma(G.w1y, v[xw], fma(-G.w1x, v[yw], fma(G.azz, v[zw], fma(G.kmt, v[zmt], fma(G.cw, vaug, sz)))));
+#pragma unroll
+  for (int i = 0; i < NC; i++) {
+    const int xc = 1 + i, yc = NT + 1 + i, zc = 2 * NT + 1 + i;
+    y[xc] = fma(G.kw[i], v[xw], fma(G.b[i], v[xc], fma(-G.o[i], v[yc], -G.w1y * v[zc])));
+    y[yc] = fma(G.kw[i], v[yw], fma(G.o[i], v[xc], fma(G.b[i], v[yc], G.w1x * v[zc])));
+    y[zc] = fma(G.kw[i], v[zw], fma(G.w1y, v[xc], fma(-G.w1x, v[yc], fma(G.c[i], v[zc], G.cc[i] * vaug))));
+  }
+  y[zmt] = fma(G.kwmt, v[zw], fma(G.dmt, v[zmt], G.cmt * vaug));
+}
+
+// Per-event coefficients of one tangent's generator derivative dX (already times dt 2^-s).
+struct TanCo {
+  double c0, c1, c2;
+};
+
+// y += dX v for tangent kind `kind` (pool `ix`), matrix.py:238-293 differentiated by hand.
+template <int NC>
+__device__ __forceinline__ void apply_dX(int kind, int ix, const TanCo& t, const double* dwp, const double* v,
+                                         double vaug, double* y) {
+  constexpr int NT = 1 + NC;
+  constexpr int xw = 0, yw = NT, zw = 2 * NT, zmt = 3 * NT;
+  switch (kind) {
+    case TK_W_R1:
+      y[zw] = fma(-t.c0, v[zw], fma(t.c0, vaug, y[zw]));
+      break;
+    case TK_W_R2:
+      y[xw] = fma(-t.c0, v[xw], y[xw]);
+      y[yw] = fma(-t.c0, v[yw], y[yw]);
+      break;
+    case TK_MT_K:
+      y[zw] = fma(-t.c0, v[zw], fma(t.c1, v[zmt], y[zw]));
+      y[zmt] = fma(t.c0, v[zw], fma(-t.c1, v[zmt], y[zmt]));
+      break;
+    case TK_MT_F:
+      y[zw] = fma(-t.c0, v[zw], y[zw]);
+      y[zmt] = fma(t.c0, v[zw], fma(t.c1, vaug, y[zmt]));
+      break;
+    case TK_MT_R1:
+      y[zmt] = fma(-t.c0, v[zmt], fma(t.c1, vaug, y[zmt]));
+      break;
+    case TK_MT_DW:
+    case TK_MT_R2:
+      y[zmt] = fma(t.c2, v[zmt], y[zmt]);
+      break;
+    case TK_B0INH:
+      // d off_w = c0, d o_i = -c0, d w_mt folded into c2
+      y[xw] = fma(t.c0, v[yw], y[xw]);
+      y[yw] = fma(-t.c0, v[xw], y[yw]);
+#pragma unroll
+      for (int i = 0; i < NC; i++) {
+        y[1 + i] = fma(t.c0, v[NT + 1 + i], y[1 + i]);
+        y[NT + 1 + i] = fma(-t.c0, v[1 + i], y[NT + 1 + i]);
+      }
+      y[zmt] = fma(t.c2, v[zmt], y[zmt]);
+      break;
+    case TK_B0:
+    case TK_GAMMA:
+      // d off_w = c0, d o_i = dw_i c1 - c0
+      y[xw] = fma(t.c0, v[yw], y[xw]);
+      y[yw] = fma(-t.c0, v[xw], y[yw]);
+#pragma unroll
+      for (int i = 0; i < NC; i++) {
+        double doi = fma(dwp[i], t.c1, -t.c0);
+        y[1 + i] = fma(-doi, v[NT + 1 + i], y[1 + i]);
+        y[NT + 1 + i] = fma(doi, v[1 + i], y[NT + 1 + i]);
+      }
+      y[zmt] = fma(t.c2, v[zmt], y[zmt]);
+      break;
+    case TK_RELB1:
+      // d w1x = c0, d w1y = c1
+      y[xw] = fma(-t.c1, v[zw], y[xw]);
+      y[yw] = fma(t.c0, v[zw], y[yw]);
+      y[zw] = fma(t.c1, v[xw], fma(-t.c0, v[yw], y[zw]));
+#pragma unroll
+      for (int i = 0; i < NC; i++) {
+        const int xc = 1 + i, yc = NT + 1 + i, zc = 2 * NT + 1 + i;
+        y[xc] = fma(-t.c1, v[zc], y[xc]);
+        y[yc] = fma(t.c0, v[zc], y[yc]);
+        y[zc] = fma(t.c1, v[xc], fma(-t.c0, v[yc], y[zc]));
+      }
+      y[zmt] = fma(t.c2, v[zmt], y[zmt]);
+      break;
+    default:
+      break;
+  }
+  if (kind >= TK_C_K) {
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+      if (i != ix) continue;
+      const int xc = 1 + i, yc = NT + 1 + i, zc = 2 * NT + 1 + i;
+      switch (kind) {
+        case TK_C_K:  // d k = 1 (c1), d kw = f (c0), d diag = -1, d water diag = -f
+          y[xw] = fma(-t.c0, v[xw], fma(t.c1, v[xc], y[xw]));
+          y[yw] = fma(-t.c0, v[yw], fma(t.c1, v[yc], y[yw]));
+          y[zw] = fma(-t.c0, v[zw], fma(t.c1, v[zc], y[zw]));
+          y[xc] = fma(t.c0, v[xw], fma(-t.c1, v[xc], y[xc]));
+          y[yc] = fma(t.c0, v[yw], fma(-t.c1, v[yc], y[yc]));
+          y[zc] = fma(t.c0, v[zw], fma(-t.c1, v[zc], y[zc]));
+          break;
+        case TK_C_F:  // d kw = k (c0), C_zc = r1c f -> d = r1c (c1)
+          y[xw] = fma(-t.c0, v[xw], y[xw]);
+          y[yw] = fma(-t.c0, v[yw], y[yw]);
+          y[zw] = fma(-t.c0, v[zw], y[zw]);
+          y[xc] = fma(t.c0, v[xw], y[xc]);
+          y[yc] = fma(t.c0, v[yw], y[yc]);
+          y[zc] = fma(t.c0, v[zw], fma(t.c1, vaug, y[zc]));
+          break;
+        case TK_C_DW:  // d o_i = c0
+          y[xc] = fma(-t.c0, v[yc], y[xc]);
+          y[yc] = fma(t.c0, v[xc], y[yc]);
+          break;
+        case TK_C_R1:
+          y[zc] = fma(-t.c0, v[zc], fma(t.c1, vaug, y[zc]));
+          break;
+        case TK_C_R2:
+          y[xc] = fma(-t.c0, v[xc], y[xc]);
+          y[yc] = fma(-t.c0, v[yc], y[yc]);
+          break;
+        default:
+          break;
+      }
+    }
+  }
+}
+
+// Per-team cache of the MT lineshape, keyed on the event's RF frequency and amplitude presence.
+struct MtCache {
+  double fr;
+  MtShape s;
+};
+
+// Builds the generator of one event (times `sc` = dt 2^-s) and, for every tangent, its coefficients.
+// Returns the unscaled augmented 1-norm ||aug dt||_1 in l1.
+template <int NC, int PT>
+__device__ void build_gen(const double* prm, const Ev& e, MtCache& mc, const int* kind, const int* kidx,
+                          Gen<NC>& G, TanCo* tc, double& l1, double& sc_out, double nan_l1, int& m_out,
+                          int& s_out, bool& is_nan) {
+  using I = PrmIdx<NC>;
+  const double r1w = prm[0], r2w = prm[1], kmt = prm[3], fmt = prm[4], dwmt = prm[5], r1mt = prm[6], r2mt = prm[7];
+  const int ls = (int)prm[8];
+  const double b0 = prm[9], gam = prm[10], b0i = prm[11], relb1 = prm[12];
+  const double L = prm[I::L], ks = prm[I::KS], kwmt = prm[I::KWMT], dw0 = prm[I::DW0];
+  const double* kk = prm + 13;
+  const double* ff = prm + 13 + NC;
+  const double* dw = prm + 13 + 2 * NC;
+  const double* r1c = prm + 13 + 3 * NC;
+  const double* r2c = prm + 13 + 4 * NC;
+  const double twopi = 6.283185307179586;
+  const double w1 = twopi * e.amp * relb1;
+  const double w1x = w1 * e.cph, w1y = w1 * e.sph;
+  const double eff = twopi * e.fr - dw0;
+  const double off_mt = eff - dwmt * 1e-6 * L;
+  const bool mt_on = fmt > 1e-9 && ls != 0;
+  double wmt = 0.0;
+  MtShape sh{0.0, 0.0, 0.0, 0.0};
+  if (mt_on && w1 != 0.0) {
+    if (e.fr != mc.fr) {
+      mc.fr = e.fr;
+      mc.s = mt_shape(ls, off_mt, 1.0 / r2mt, L * 1e-6);
+    }
+    sh = mc.s;
+    wmt = w1 * w1 * sh.g;
+  }
+  // the 1-norm of the augmented generator (columns), unscaled by 2^-s
+  const double dt = e.dt;
+  double axx = -r2w - ks, azz = -r1w - ks - kwmt, dmt = -r1mt - kmt - wmt;
+  double skw = 0.0;
+#pragma unroll
+  for (int i = 0; i < NC; i++) skw += fabs(kk[i] * ff[i]);
+  double cx = fabs(axx) + fabs(eff) + fabs(w1y) + skw;
+  double cy = fabs(eff) + fabs(axx) + fabs(w1x) + skw;
+  double cz = fabs(w1y) + fabs(w1x) + fabs(azz) + skw + fabs(kwmt);
+  double cmx = fabs(kmt) + fabs(dmt);
+  double caug = fabs(r1w) + fabs(r1mt * fmt);
+  double n1 = fmax(fmax(cx, cy), fmax(cz, cmx));
+#pragma unroll
+  for (int i = 0; i < NC; i++) {
+    double oi = dw[i] * 1e-6 * L - eff;
+    double bi = -r2c[i] - kk[i], ci = -r1c[i] - kk[i];
+    n1 = fmax(n1, fabs(kk[i]) + fabs(bi) + fabs(oi) + fabs(w1y));
+    n1 = fmax(n1, fabs(kk[i]) + fabs(oi) + fabs(bi) + fabs(w1x));
+    n1 = fmax(n1, fabs(kk[i]) + fabs(w1y) + fabs(w1x) + fabs(ci));
+    caug += fabs(r1c[i] * ff[i]);
+  }
+  n1 = fmax(n1, caug);
+  l1 = n1 * dt;
+  is_nan = !(l1 < nan_l1);
+  // Taylor degree and squarings
+  int m, s;
+  if (l1 <= kTheta[kMMax]) {
+    s = 0;
+    m = 1;
+    while (m < kMMax && kTheta[m] < l1) m++;
+    if (l1 == 0.0) m = 1;
+  } else {
+    s = (int)ceil(log2(l1 / kTheta[kMMax]));
+    if (s < 0) s = 0;
+    m = kMMax;
+    while (s > 0 && l1 * ldexp(1.0, -s) > kTheta[m]) s++;  // guard rounding of log2
+  }
+  if (is_nan) s = 0;
+  m_out = m;
+  s_out = s;
+  const double sc = dt * ldexp(1.0, -s);
+  sc_out = sc;
+  G.axx = axx * sc;
+  G.offw = eff * sc;
+  G.w1x = w1x * sc;
+  G.w1y = w1y * sc;
+  G.azz = azz * sc;
+  G.cw = r1w * sc;
+#pragma unroll
+  for (int i = 0; i < NC; i++) {
+    G.b[i] = (-r2c[i] - kk[i]) * sc;
+    G.o[i] = (dw[i] * 1e-6 * L - eff) * sc;
+    G.c[i] = (-r1c[i] - kk[i]) * sc;
+    G.k[i] = kk[i] * sc;
+    G.kw[i] = kk[i] * ff[i] * sc;
+    G.cc[i] = r1c[i] * ff[i] * sc;
+  }
+  G.dmt = dmt * sc;
+  G.kwmt = kwmt * sc;
+  G.kmt = kmt * sc;
+  G.cmt = r1mt * fmt * sc;
+  (void)b0;
+  (void)gam;
+  // tangent coefficients
+#pragma unroll
+  for (int p = 0; p < PT; p++) {
+    TanCo t{0.0, 0.0, 0.0};
+    const int ix = kidx[p];
+    switch (kind[p]) {
+      case TK_W_R1:
+      case TK_W_R2:
+        t.c0 = sc;
+        break;
+      case TK_MT_K:
+        t.c0 = fmt * sc;
+        t.c1 = sc;
+        break;
+      case TK_MT_F:
+        t.c0 = kmt * sc;
+        t.c1 = r1mt * sc;
+        break;
+      case TK_MT_R1:
+        t.c0 = sc;
+        t.c1 = fmt * sc;
+        break;
+      case TK_MT_DW:  // d off_mt = -1e-6 L
+        if (mt_on) t.c2 = -(w1 * w1 * sh.g_off * (-1e-6 * L)) * sc;
+        break;
+      case TK_MT_R2:  // t2 = 1 / r2
+        if (mt_on) t.c2 = -(w1 * w1 * sh.g_t2 * (-1.0 / (r2mt * r2mt))) * sc;
+        break;
+      case TK_B0INH:
+        t.c0 = -1e-6 * L * sc;
+        if (mt_on) t.c2 = -(w1 * w1 * sh.g_off * (-1e-6 * L)) * sc;
+        break;
+      case TK_B0:
+      case TK_GAMMA: {
+        const double dL = (kind[p] == TK_B0) ? gam * 1e6 : 1e6 * b0;
+        t.c0 = -dL * b0i * 1e-6 * sc;
+        t.c1 = 1e-6 * dL * sc;
+        if (mt_on) {
+          double doff = -1e-6 * dL * (b0i + dwmt);
+          t.c2 = -(w1 * w1 * (sh.g_off * doff + sh.g_w0 * 1e-6 * dL)) * sc;
+        }
+      } break;
+      case TK_RELB1: {
+        const double dw1 = twopi * e.amp;
+        t.c0 = dw1 * e.cph * sc;
+        t.c1 = dw1 * e.sph * sc;
+        if (mt_on) t.c2 = -(2.0 * w1 * dw1 * sh.g) * sc;
+      } break;
+      case TK_C_K:
+        t.c0 = ff[ix] * sc;
+        t.c1 = sc;
+        break;
+      case TK_C_F:
+        t.c0 = kk[ix] * sc;
+        t.c1 = r1c[ix] * sc;
+        break;
+      case TK_C_DW:
+        t.c0 = 1e-6 * L * sc;
+        break;
+      case TK_C_R1:
+        t.c0 = sc;
+        t.c1 = ff[ix] * sc;
+        break;
+      case TK_C_R2:
+        t.c0 = sc;
+        break;
+      default:
+        break;
+    }
+    tc[p] = t;
+  }
+}
+
+// Truncated Taylor series acc = T_m(X) v0 with tangents dacc = d(T_m(X) v0) (v0's own tangent dv0).
+template <int NC, int PT>
+__device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
+                                       const double* dwp, int m, const double* v0, double vaug0,
+                                       const double (*dv0)[3 * (1 + NC) + 1], double* acc,
+                                       double (*dacc)[3 * (1 + NC) + 1]) {
+  constexpr int N = 3 * (1 + NC) + 1;
+  double t[N], dtn[PT > 0 ? PT : 1][N];
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    t[i] = v0[i];
+    acc[i] = v0[i];
+  }
+#pragma unroll
+  for (int p = 0; p < PT; p++)
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      dtn[p][i] = dv0 ? dv0[p][i] : 0.0;
+      dacc[p][i] = dtn[p][i];
+    }
+  double taug = vaug0;
+  for (int k = 1; k <= m; k++) {
+    const double ik = kInv[k];
+    double y[N];
+    apply_X<NC>(G, t, taug, y);
+#pragma unroll
+    for (int p = 0; p < PT; p++) {
+      double dy[N];
+      apply_X<NC>(G, dtn[p], 0.0, dy);
+      apply_dX<NC>(kind[p], kidx[p], tc[p], dwp, t, taug, dy);
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        dtn[p][i] = dy[i] * ik;
+        dacc[p][i] += dtn[p][i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      t[i] = y[i] * ik;
+      acc[i] += t[i];
+    }
+    taug = 0.0;
+  }
+}
+
+// out = A B for affine propagators in column layout (lane j holds column j, rows 0..N-1):
+// out[:, j] = sum_k A[:, k] B[k, j] + A[:, N] [j == N]; tangents by the product rule.
+template <int NC, int PT, int TEAM>
+__device__ __forceinline__ void mul_cols(const double* A, const double (*dA)[3 * (1 + NC) + 1], double* B,
+                                         double (*dB)[3 * (1 + NC) + 1], int j) {
+  constexpr int N = 3 * (1 + NC) + 1;
+  double o[N], od[PT > 0 ? PT : 1][N];
+  const bool aff = (j == N);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    o[i] = aff ? A[i] : 0.0;
+#pragma unroll
+    for (int p = 0; p < PT; p++) od[p][i] = aff ? dA[p][i] : 0.0;
+  }
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const double bk = B[k];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      const double a = bcast<TEAM, k>(A[i]);
+      o[i] = fma(a, bk, o[i]);
+#pragma unroll
+      for (int p = 0; p < PT; p++) od[p][i] = fma(a, dB[p][k], od[p][i]);
+    }
+#pragma unroll
+    for (int p = 0; p < PT; p++)
+#pragma unroll
+      for (int i = 0; i < N; i++) od[p][i] = fma(bcast<TEAM, k>(dA[p][i]), bk, od[p][i]);
+  });
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    B[i] = o[i];
+#pragma unroll
+    for (int p = 0; p < PT; p++) dB[p][i] = od[p][i];
+  }
+}
+
+// M <- A M (state uniform over the team, A in column layout), with tangents.
+template <int NC, int PT, int TEAM>
+__device__ __forceinline__ void apply_cols(const double* A, const double (*dA)[3 * (1 + NC) + 1], double* M,
+                                           double (*dM)[3 * (1 + NC) + 1], int j) {
+  constexpr int N = 3 * (1 + NC) + 1;
+  double mj = (j == N) ? 1.0 : 0.0;
+  double dmj[PT > 0 ? PT : 1];
+#pragma unroll
+  for (int p = 0; p < PT; p++) dmj[p] = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    if (j == i) {
+      mj = M[i];
+#pragma unroll
+      for (int p = 0; p < PT; p++) dmj[p] = dM[p][i];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PT; p++)
+#pragma unroll
+    for (int i = 0; i < N; i++) dM[p][i] = team_sum<TEAM>(fma(dA[p][i], mj, A[i] * dmj[p]));
+#pragma unroll
+  for (int i = 0; i < N; i++) M[i] = team_sum<TEAM>(A[i] * mj);
+}
+
+// The propagator of one event in column layout: Taylor column + s squarings.
+template <int NC, int PT, int TEAM>
+__device__ __forceinline__ void event_cols(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
+                                           const double* dwp, int m, int s, bool is_nan, int j, double* R,
+                                           double (*dR)[3 * (1 + NC) + 1]) {
+  constexpr int N = 3 * (1 + NC) + 1;
+#pragma unroll
+  for (int i = 0; i < N; i++) R[i] = (i == j) ? 1.0 : 0.0;
+  taylor<NC, PT>(G, tc, kind, kidx, dwp, m, R, (j == N) ? 1.0 : 0.0, nullptr, R, dR);  // column j of T_m(X)
+  // squarings: mul_cols finishes the product before it writes back, so R may be both factors
+  for (int q = 0; q < s; q++) mul_cols<NC, PT, TEAM>(R, dR, R, dR, j);
+  if (is_nan) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      R[i] = __builtin_nan("");
+#pragma unroll
+      for (int p = 0; p < PT; p++) dR[p][i] = __builtin_nan("");
+    }
+  }
+}
+
+}  // namespace bmc

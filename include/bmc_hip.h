@@ -145,6 +145,33 @@ int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const 
                double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev, void* stream);
 
 /*
+ * Reverse mode of bmc_simulate with respect to the events' RF amplitudes and RF frequencies: given
+ * cotangents on Z = state[jac_component] and on the Jacobian J[v, o, p] that bmc_simulate returns with
+ * the same tangents, adds dL/d rf_amp[o, t] and dL/d rf_freq[o, t], summed over voxels, into the
+ * outputs (the caller clears them).
+ *
+ * Replaces: jax.value_and_grad through jacfwd of the scan in optimize_design
+ * (design/optimize.py:80-93, over design/crlb.py:38-58 and simulation/simulate.py:22-132).
+ *
+ * The forward scan is stored at every K = ceil(sqrt(T))-th event and each segment is recomputed in the
+ * reverse sweep (sqrt(T) checkpoints); bmc_adjoint_workspace_size gives the workspace it needs
+ * (device memory the caller provides; its contents on entry do not matter).
+ *   z_cot_dev    [V, O] float64 or NULL      cotangent of Z
+ *   jac_cot_dev  [V, O, P] float64 or NULL   cotangent of J (J includes tangent_scale)
+ *   grad_amp_dev [O, T] float64 or NULL      += dL / d rf_amp   (Hz)
+ *   grad_freq_dev[O, T] float64 or NULL      += dL / d rf_freq  (Hz)
+ * Supported: n_cest <= 2, at most 2 tangents, and only tangents whose generator derivative does not
+ * depend on the RF (water, CEST and MT rates and fractions, CEST shifts); other slots return
+ * BMC_ENOTSUP. dm0_dev as in bmc_simulate.
+ */
+int bmc_adjoint_workspace_size(const bmc_plan* plan, int n_voxels, int n_cest, int n_tangents, int64_t* bytes);
+int bmc_simulate_adjoint(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                         const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
+                         const double* tangent_scale, int jac_component, const double* z_cot_dev,
+                         const double* jac_cot_dev, double* grad_amp_dev, double* grad_freq_dev,
+                         void* workspace_dev, int64_t workspace_bytes, void* stream);
+
+/*
  * The learned LM proposal: an MLP trained and evaluated with hand-written BF16 MFMA GEMMs (FP32
  * accumulate), GELU hidden layers, a linear output, mean-squared-error loss and Adam. No counterpart
  * in the reference; its output is the LM start x0 (plain x0 remains the fallback).
