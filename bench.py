@@ -62,6 +62,118 @@ def workload(n_vox, seed=2):
     return prm, m0, ev, nc
 
 
+def design_workload(n_truths=1024, seed=4):
+    """The adjoint protocol-gradient problem: 3 pools (water, amide +3.5 ppm, SuperLorentzian MT at
+    -2.5 ppm), a shaped pulse of 256 samples repeated 40 times (20 ms pulses, 10 ms gaps) after a 1 s
+    recovery, 31 offsets, CRLB cost of (k, f) averaged over n_truths seeded parameter sets."""
+    from bmc_jax.fitting import FitSpec, FreeParam
+    from bmc_jax.fitting.experiment import Experiment
+    from bmc_jax.sequences import pulsed_train
+    from bmc_jax.sequences.saturation import hz_per_ppm
+    from bmc_jax.simulation.params import SimParams
+    rng = np.random.default_rng(seed)
+    f32 = lambda v: np.float32(v)
+    sp = SimParams(water_r1=f32(1 / 1.3), water_r2=f32(1 / 0.075), water_f=f32(1.0), cest_k=np.array([50.0], np.float32),
+                   cest_f=np.array([2e-3], np.float32), cest_dw=np.array([3.5], np.float32),
+                   cest_r1=np.array([1 / 1.3], np.float32), cest_r2=np.array([10.0], np.float32), mt_k=f32(30.0),
+                   mt_f=f32(0.1), mt_dw=f32(-2.5), mt_r1=f32(1.0), mt_r2=f32(1e5), mt_lineshape=np.int32(2),
+                   b0=f32(3.0), gamma=f32(267.5153), b0_inhom=f32(0.0), rel_b1=f32(1.0))
+    m0 = np.zeros(7, np.float32)
+    m0[4], m0[5], m0[6] = 1.0, 2e-3, 0.1
+    ppm = np.linspace(-5.0, 5.0, 31)
+    ev = pulsed_train(ppm, b1_peak_uT=2.0, n_pulses=40, pulse_s=0.02, n_samples=256, gap_s=0.01, recovery_s=1.0,
+                      b0_T=3.0, dtype=np.float64)
+    typ = np.asarray(ev.type)
+    isrf = typ[0] == 1
+    sidx = np.where(isrf, (np.cumsum(isrf) - 1) % 256, 0)
+    shape0 = np.asarray(ev.rf_amp)[0, np.flatnonzero(isrf)[:256]]
+    d0 = np.concatenate([shape0, ppm])
+    spec = FitSpec([FreeParam("k", "cest_k", 0, 100.0, 5.0, 20000.0, 100.0),
+                    FreeParam("f", "cest_f", 0, 0.001, 1e-6, 0.05, 2e-3)])
+    truths = np.stack([np.exp(rng.uniform(np.log(5.0), np.log(5000.0), n_truths)),
+                       rng.uniform(1e-3, 0.02, n_truths)], axis=1)
+
+    def build(d):
+        import torch
+        shape, offs = d[:256], d[256:]
+        isrf_t = torch.as_tensor(isrf)
+        amp = torch.where(isrf_t, shape[torch.as_tensor(sidx)], torch.zeros((), dtype=d.dtype))
+        amp = amp[None, :].expand(len(ppm), -1)
+        freq = torch.where(isrf_t[None, :], offs[:, None] * hz_per_ppm(3.0), torch.zeros((), dtype=d.dtype))
+        events = ev._replace(rf_amp=amp, rf_freq=freq)
+        return Experiment(sp, events, m0, 4)
+
+    return build, d0, spec, truths, ev
+
+
+def bench_design(args):
+    import torch
+    from bmc_jax import _flops
+    from bmc_jax.design.adjoint import design_cost
+    build, d0, spec, truths, ev = design_workload(args.truths)
+    d = torch.tensor(d0, dtype=torch.float64, requires_grad=True)
+
+    def step():
+        d.grad = None
+        c = design_cost(build(d), spec, 0.002, truths, np.array([0, 1]), dtype=np.float32)
+        c.backward()
+        return float(c.detach()), d.grad
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        c, g = step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    assert np.isfinite(c) and torch.isfinite(g).all().item()
+    # the adjoint launch alone, timed on its stream
+    from bmc_jax import _native
+    from bmc_jax.design.adjoint import scan_jacobian
+    Z, J = scan_jacobian(build(d.detach()), spec, truths, dtype=np.float32)
+    gz, gj = torch.zeros_like(Z, dtype=torch.float64), torch.ones_like(J, dtype=torch.float64)
+    from bmc_jax.design.adjoint import ScanProblem  # noqa: F401
+    ev_np = {k: np.asarray(getattr(ev, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    plan = _native.Plan(ev_np["type"], ev_np["dt"], ev_np["rf_amp"], ev_np["rf_phase"], ev_np["rf_freq"])
+    from bmc_jax.simulation.params import pack
+    from bmc_jax.fitting.params import write_params
+    from bmc_jax.simulation.voxels import tile_params
+    exp = build(d.detach())
+    pb = write_params(tile_params(exp.params, truths.shape[0]), truths / spec.scales, spec)
+    prm = _native.device_array(pack(pb, truths.shape[0], dtype=np.float32), torch.float32)
+    m0 = _native.device_array(np.broadcast_to(exp.m0, (truths.shape[0], 7)).astype(np.float32), torch.float32)
+    slots = [_native.slot_index("cest_k", 0, 1), _native.slot_index("cest_f", 0, 1)]
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    _native.simulate_adjoint(plan, prm, 1, _native.BMC_F32, m0, slots=slots, scales=[1.0, 1.0], jc=4,
+                             z_cot=gz, jac_cot=gj)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    adj_ms = e0.elapsed_time(e1)
+    sample = np.random.default_rng(0).choice(truths.shape[0], size=min(8, truths.shape[0]), replace=False)
+    flops = float(_flops.adjoint_flops(pack(pb, truths.shape[0])[sample].astype(np.float64), 1, ev_np, slots).mean()) \
+        * truths.shape[0]
+    achieved = flops / (adj_ms * 1e-3) / 1e12
+    ms = 1e3 * float(np.mean(times))
+    out = {"metric": "protocol-design gradients/s (CRLB of k, f; adjoint)", "value": 1e3 / ms,
+           "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 (float32 I/O)",
+           "data": "synthetic (seeded parameter sets)",
+           "config": {"workload": "adjoint protocol gradient: 3 pools (water, amide, SL MT), 256-sample shaped pulse "
+                                  "x 40 + gaps + 1 s recovery (10,283 events), 31 offsets, "
+                                  f"{truths.shape[0]} parameter sets, d = 256 amplitudes + 31 offsets",
+                      "truths": int(truths.shape[0]), "offsets": 31, "events_per_offset": int(ev_np["dt"].shape[1])},
+           "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (adjoint kernel)",
+                        "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "flops_per_launch": flops,
+                        "avg_launch_ms": adj_ms},
+           "cost": c}
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(prm, m0, ev, nc, slots, budget_s=15.0):
     """The C oracle on this host's cores: whole spectra of the first voxels, as many as fit the budget."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -90,7 +202,8 @@ def main():
     ap.add_argument("--voxels", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flop-sample", type=int, default=16)
-    ap.add_argument("--workload", choices=("pulsed", "nn"), default="pulsed",
+    ap.add_argument("--truths", type=int, default=1024)
+    ap.add_argument("--workload", choices=("pulsed", "nn", "design"), default="pulsed",
                     help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
     ap.add_argument("--nn-train", type=int, default=2_000_000)
     ap.add_argument("--nn-test", type=int, default=262_144)
@@ -105,6 +218,10 @@ def main():
     from bmc_jax import _engine, _flops, _native
     from bmc_jax.simulation.events import SimEvent
 
+    if args.workload == "design":
+        _native.torch_cuda()
+        bench_design(args)
+        return
     if args.workload == "nn":
         from bmc_jax import learned
         _native.torch_cuda()

@@ -64,9 +64,11 @@ def _gen_l1(prm, nc, dt, amp, cph, sph, fr):
         for v in np.flatnonzero(on):
             off = eff[v] - dwmt[v, 0] * 1e-6 * L[v, 0]
             t2 = 1.0 / r2mt[v, 0]
-            for j in np.flatnonzero(w1[v] != 0):
-                g = lorentzian(off[j], t2) if ls[v, 0] == 1 else super_lorentzian(off[j], t2, L[v, 0] * 1e-6)
-                wmt[v, j] = w1[v, j] ** 2 * g
+            js = np.flatnonzero(w1[v] != 0)
+            uniq, inv = np.unique(off[js], return_inverse=True)
+            g = np.array([lorentzian(u, t2) if ls[v, 0] == 1 else super_lorentzian(u, t2, L[v, 0] * 1e-6)
+                          for u in uniq])
+            wmt[v, js] = w1[v, js] ** 2 * g[inv]
     axx, azz, dmt = -r2w - ks, -r1w - ks - kwmt, -r1mt - kmt - wmt
     skw = np.abs(prm[:, 13:13 + nc] * prm[:, 13 + nc:13 + 2 * nc]).sum(1)[:, None]
     n1 = np.maximum.reduce([np.abs(axx) + np.abs(eff) + np.abs(w1y) + skw, np.abs(eff) + np.abs(axx) + np.abs(w1x) + skw,
@@ -156,3 +158,44 @@ def reference_rule(l1):
     s = 0 if l1 <= 0 else max(0, math.floor(math.log2(l1 / 3.925724783138660)))
     m = 3 if l1 < 0.4258730016922831 else 5 if l1 < 1.880152677804762 else 7
     return m, s
+
+
+def adjoint_flops(prm, nc, events, slots, n_q=2):
+    """Useful float64 flops of one bmc_simulate_adjoint call, per voxel of prm [V, S] (summed over rows).
+
+    Per live event: the forward step twice (checkpoint sweep and segment recompute; all P tangents in
+    one pass) and, for each of the n_q RF directions, the second-order columns (Taylor terms of the
+    state, P tangents, the RF direction and P mixed terms; squarings by the product rule) and the
+    adjoint dot products. Idle lanes of a team are not counted.
+    """
+    ev = {k: np.asarray(getattr(events, k) if not isinstance(events, dict) else events[k])
+          for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    N = 3 * (1 + nc) + 1
+    NA = N + 1
+    FX = 29 * nc + 27
+    P = len(slots)
+    fdx = sum(dx_flops(sl, nc) for sl in slots)
+    dxq = (2 * (5 + 4 * nc), 2 * (3 + 2 * nc))[:n_q]           # rf_amp (rel_b1 pattern), rf_freq (b0 shift)
+    term = FX * (1 + P) + fdx + 2 * N * (1 + P)
+    mul = NA * 2 * N * N * (1 + 2 * P)
+    app = NA * (N + 3 * P * N) + N * (1 + P) * (NA - 1)
+    sq2 = NA * 2 * (3 + 6 * P) * N * N
+    dots = NA * 2 * (2 + 4 * P) * N
+    V = prm.shape[0]
+    total = np.zeros(V)
+    for o in range(ev["dt"].shape[0]):
+        dt = ev["dt"][o].astype(np.float64)
+        sel = np.flatnonzero(dt > 1e-9)
+        if sel.size == 0:
+            continue
+        ph = ev["rf_phase"][o, sel].astype(np.float64)
+        l1 = _gen_l1(prm, nc, dt[sel], ev["rf_amp"][o, sel].astype(np.float64), np.cos(ph), np.sin(ph),
+                     ev["rf_freq"][o, sel].astype(np.float64))
+        m, s = degree_and_squarings(l1)
+        fwd = np.where(s == 0, m * term, NA * m * term + s * mul + app)
+        adj = 0.0
+        for dq in dxq:
+            term2 = FX * (2 + 2 * P) + fdx + dq + P * dq + fdx + 2 * N * (2 + 2 * P)
+            adj = adj + NA * m * term2 + s * sq2 + dots
+        total += (2 * fwd + adj).sum(axis=1)
+    return total

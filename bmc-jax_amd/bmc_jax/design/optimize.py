@@ -1,11 +1,15 @@
 """A protocol with a lower Cramer-Rao bound, by gradient descent (src/bmc_jax/design/optimize.py:31-113).
 
 cost(d) = mean over truths of sum over targets of var_j / x_j^2, with d = lo + (hi - lo) sigmoid(u) and
-Adam on u. `build(d)` is arbitrary caller code here (NumPy, not a traced function), so the gradient
-of the cost with respect to d is taken by central differences of the GPU-evaluated cost, one pair of
-Fisher evaluations per design entry.
+Adam on u. How the gradient of the cost with respect to d is taken depends on `build(d)`:
+  * `build` written with torch operations, returning an Experiment whose events' rf_amp / rf_freq are
+    tensors that depend on d: reverse mode through torch and the adjoint kernel (design/adjoint.py),
+    one forward and one adjoint launch per step whatever the length of d -- the reference's
+    value_and_grad (optimize.py:90-93);
+  * any other `build` (NumPy): central differences of the GPU-evaluated cost, two cost evaluations
+    per design entry.
 """
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 from typing import Callable, Optional, Sequence
 
 import numpy as np
@@ -24,10 +28,30 @@ class DesignResult:
     rel_sd_end: np.ndarray
 
 
+def _numpy_experiment(exp):
+    """The experiment with its event tensors (if any) as NumPy arrays."""
+    ev = exp.events
+    conv = lambda x: x.detach().cpu().numpy() if hasattr(x, "detach") else x
+    return replace(exp, events=type(ev)(*[conv(f) for f in ev]))
+
+
+def _build_numpy(build, d):
+    """build(d) for the bound reports: tried with a NumPy d first, then with a float64 tensor."""
+    try:
+        return _numpy_experiment(build(np.asarray(d, float)))
+    except Exception:
+        import torch
+        return _numpy_experiment(build(torch.as_tensor(np.asarray(d, float))))
+
+
 def optimize_design(build: Callable, spec, d0, lo, hi, sigma: float, truths, targets: Optional[Sequence[str]] = None,
                     n_steps: int = 200, lr: float = 0.05, callback: Optional[Callable] = None,
-                    fd_step: float = 1e-4) -> DesignResult:
-    """Lower the bound of the target parameters by changing the design (optimize.py:43-113)."""
+                    fd_step: float = 1e-4, gradient: str = "auto") -> DesignResult:
+    """Lower the bound of the target parameters by changing the design (optimize.py:43-113).
+
+    gradient: "adjoint" (build must use torch operations), "fd" (central differences), or "auto"
+    (adjoint when build(d) for a tensor d returns differentiable event tensors, else differences).
+    """
     names = [s.name for s in spec.slots]
     targets = tuple(names if targets is None else targets)
     idx = np.asarray([names.index(t) for t in targets])
@@ -37,7 +61,7 @@ def optimize_design(build: Callable, spec, d0, lo, hi, sigma: float, truths, tar
         raise ValueError("d0, lo and hi need the same shape, with lo < d0 < hi")
 
     def cost_of(d):
-        exp = build(d)
+        exp = _numpy_experiment(build(d))
         total = 0.0
         for x in truths:
             F, _ = fisher_information(exp, spec, x, sigma)
@@ -50,7 +74,7 @@ def optimize_design(build: Callable, spec, d0, lo, hi, sigma: float, truths, tar
     p0 = (d0 - lo) / (hi - lo)
     u = np.log(p0 / (1.0 - p0))
 
-    def value_and_grad(u):
+    def value_and_grad_fd(u):
         c = cost_of(to_d(u))
         g = np.zeros_like(u)
         for j in range(u.size):
@@ -58,6 +82,29 @@ def optimize_design(build: Callable, spec, d0, lo, hi, sigma: float, truths, tar
             e[j] = fd_step
             g[j] = (cost_of(to_d(u + e)) - cost_of(to_d(u - e))) / (2 * fd_step)
         return c, g
+
+    def value_and_grad_adjoint(u):
+        import torch
+        from .adjoint import design_cost
+        ut = torch.tensor(u, dtype=torch.float64, requires_grad=True)
+        lo_t, hi_t = torch.as_tensor(lo), torch.as_tensor(hi)
+        exp = build(lo_t + (hi_t - lo_t) * torch.sigmoid(ut))
+        c = design_cost(exp, spec, sigma, truths, idx)
+        c.backward()
+        return float(c.detach()), ut.grad.numpy().copy()
+
+    if gradient not in ("auto", "adjoint", "fd"):
+        raise ValueError(f"gradient must be 'auto', 'adjoint' or 'fd', got {gradient!r}")
+    use_adjoint = gradient == "adjoint"
+    if gradient == "auto":
+        try:
+            import torch
+            from .adjoint import is_differentiable
+            probe = build(torch.tensor(d0, dtype=torch.float64, requires_grad=True))
+            use_adjoint = is_differentiable(probe.events)
+        except Exception:
+            use_adjoint = False
+    value_and_grad = value_and_grad_adjoint if use_adjoint else value_and_grad_fd
 
     m, v, b1, b2, eps = np.zeros_like(u), np.zeros_like(u), 0.9, 0.999, 1e-8
     history = []
@@ -72,9 +119,15 @@ def optimize_design(build: Callable, spec, d0, lo, hi, sigma: float, truths, tar
         if callback is not None:
             callback(step, float(c), np.asarray(to_d(u)))
     design = np.asarray(to_d(u), float)
-    history.append(float(cost_of(design)))
+    if use_adjoint:
+        import torch
+        from .adjoint import design_cost
+        with torch.no_grad():
+            history.append(float(design_cost(build(torch.as_tensor(design)), spec, sigma, truths, idx)))
+    else:
+        history.append(float(cost_of(design)))
 
     def rel(d):
-        exp = build(np.asarray(d))
+        exp = _build_numpy(build, d)
         return np.array([[crlb(exp, spec, sigma, x).rel_sd[names.index(t)] for t in targets] for x in truths])
     return DesignResult(design, d0, np.asarray(history), targets, truths, rel(d0), rel(design))
