@@ -588,4 +588,154 @@ __device__ __forceinline__ void event_cols(const Gen<NC>& G, const TanCo* tc, co
   }
 }
 
+
+// Runs one offset row's program (the event scan, simulate.py:80-132) on the team's state M and its
+// tangents dM: SEQ ops event by event, BLOCK ops chained into one propagator applied `count` times.
+// prm / st0: the team's parameters and start state (+ seeds) in LDS; st_park: LDS room for the state
+// while a block is chained. traj (optional): last_M after every event at traj_base + event * N.
+template <int NC, int PT, int TEAM>
+__device__ __forceinline__ void run_row(const double* prm, const double* st0, double* st_park, const int* kind,
+                                        const int* kidx, const double* dwp, const Ev* ev_row, const Op* ops, int op0,
+                                        int op1, int reset_after_adc, double nan_l1, int lane,
+                                        double* M, double (*dM)[3 * (1 + NC) + 1], void* traj, long traj_base,
+                                        int traj_dtype) {
+  constexpr int NT = 1 + NC, N = 3 * NT + 1;
+  constexpr int PTA = PT > 0 ? PT : 1;
+  auto load_start = [&]() {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      M[i] = st0[i];
+#pragma unroll
+      for (int p = 0; p < PTA; p++) dM[p][i] = st0[(1 + p) * N + i];
+    }
+  };
+  load_start();
+
+  MtCache mc;
+  mc.fr = __builtin_nan("");
+  mc.s = MtShape{0.0, 0.0, 0.0, 0.0};
+  const Ev* evr = ev_row;
+  bool pending = false;
+
+  for (int oi = op0; oi < op1; oi++) {
+    const Op op = ops[oi];
+    if (op.kind == OP_SEQ) {
+      Ev e_next = evr[op.e0];
+      for (int ei = op.e0; ei < op.e0 + op.len; ei++) {
+        const Ev e = e_next;
+        if (ei + 1 < op.e0 + op.len) e_next = evr[ei + 1];
+        if (pending) {
+          load_start();
+          pending = false;
+        }
+        if (e.dt > 1e-9) {  // simulate.py:51
+          Gen<NC> G;
+          TanCo tc[PTA];
+          double l1, sc;
+          int m, s;
+          bool isn;
+          build_gen<NC, PT>(prm, e, mc, kind, kidx, G, tc, l1, sc, nan_l1, m, s, isn);
+          if (s == 0) {
+            taylor<NC, PT>(G, tc, kind, kidx, dwp, m, M, 1.0, dM, M, dM);  // in place
+            if (isn) {
+#pragma unroll
+              for (int i = 0; i < N; i++) {
+                M[i] = __builtin_nan("");
+#pragma unroll
+                for (int p = 0; p < PTA; p++) dM[p][i] = __builtin_nan("");
+              }
+            }
+          } else {
+            double R[N], dR[PTA][N];
+            event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
+            apply_cols<NC, PT, TEAM>(R, dR, M, dM, lane);
+          }
+        }
+        if (e.type == BMC_EV_GRADIENT || e.type == BMC_EV_SPOILER) {  // simulate.py:55-72
+#pragma unroll
+          for (int i = 0; i < 2 * NT; i++) {
+            M[i] = 0.0;
+#pragma unroll
+            for (int p = 0; p < PTA; p++) dM[p][i] = 0.0;
+          }
+        }
+        if (traj && lane == 0) {  // the trajectory: last_M after every event
+          const long base = traj_base + (long)ei * N;
+#pragma unroll
+          for (int i = 0; i < N; i++) store_real(traj, base + i, M[i], traj_dtype);
+        }
+        if (e.type == BMC_EV_ADC && reset_after_adc) pending = true;  // simulate.py:105-114
+      }
+    } else {
+      if (pending) {
+        load_start();
+        pending = false;
+      }
+      // the state is not touched while the block is chained: park it in LDS
+      double* st = st_park;
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          st[i] = M[i];
+#pragma unroll
+          for (int p = 0; p < PTA; p++) st[(1 + p) * N + i] = dM[p][i];
+        }
+      }
+      double Pi[N], dPi[PTA][N];
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        Pi[i] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+        for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
+      }
+      Ev e_next = evr[op.e0];
+      for (int ei = op.e0; ei < op.e0 + op.len; ei++) {
+        const Ev e = e_next;
+        if (ei + 1 < op.e0 + op.len) e_next = evr[ei + 1];  // the next record is in flight meanwhile
+        if (e.dt > 1e-9) {
+          Gen<NC> G;
+          TanCo tc[PTA];
+          double l1, sc;
+          int m, s;
+          bool isn;
+          build_gen<NC, PT>(prm, e, mc, kind, kidx, G, tc, l1, sc, nan_l1, m, s, isn);
+          if (s == 0) {
+            // exp(X) Pi = T_m(X) Pi: each lane runs the series on its own column of the chained
+            // propagator (homogeneous coordinate Pi[N][j] = [j == N]); no dense product is needed
+            taylor<NC, PT>(G, tc, kind, kidx, dwp, m, Pi, (lane == N) ? 1.0 : 0.0, dPi, Pi, dPi);  // in place
+            if (isn) {
+#pragma unroll
+              for (int i = 0; i < N; i++) {
+                Pi[i] = __builtin_nan("");
+#pragma unroll
+                for (int p = 0; p < PTA; p++) dPi[p][i] = __builtin_nan("");
+              }
+            }
+          } else {
+            double R[N], dR[PTA][N];
+            event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
+            mul_cols<NC, PT, TEAM>(R, dR, Pi, dPi, lane);
+          }
+        }
+        if (e.type == BMC_EV_GRADIENT || e.type == BMC_EV_SPOILER) {
+#pragma unroll
+          for (int i = 0; i < 2 * NT; i++) {
+            Pi[i] = 0.0;
+#pragma unroll
+            for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        M[i] = st[i];
+#pragma unroll
+        for (int p = 0; p < PTA; p++) dM[p][i] = st[(1 + p) * N + i];
+      }
+      for (int c = 0; c < op.count; c++) apply_cols<NC, PT, TEAM>(Pi, dPi, M, dM, lane);
+    }
+  }
+
+}
+
 }  // namespace bmc

@@ -110,142 +110,9 @@ __global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
   // state (identical on every lane of the team); the start state is reloaded from memory on a reset
   double M[N];
   double dM[PTA][N];
-  auto load_start = [&]() {
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      M[i] = st0[i];
-#pragma unroll
-      for (int p = 0; p < PTA; p++) dM[p][i] = st0[(1 + p) * N + i];
-    }
-  };
-  load_start();
-
-  MtCache mc;
-  mc.fr = __builtin_nan("");
-  mc.s = MtShape{0.0, 0.0, 0.0, 0.0};
-  const Ev* evr = a.ev + (long)o * a.T;
-  bool pending = false;
-
-  const int op0 = a.row_ptr[o], op1 = a.row_ptr[o + 1];
-  for (int oi = op0; oi < op1; oi++) {
-    const Op op = a.ops[oi];
-    if (op.kind == OP_SEQ) {
-      Ev e_next = evr[op.e0];
-      for (int ei = op.e0; ei < op.e0 + op.len; ei++) {
-        const Ev e = e_next;
-        if (ei + 1 < op.e0 + op.len) e_next = evr[ei + 1];
-        if (pending) {
-          load_start();
-          pending = false;
-        }
-        if (e.dt > 1e-9) {  // simulate.py:51
-          Gen<NC> G;
-          TanCo tc[PTA];
-          double l1, sc;
-          int m, s;
-          bool isn;
-          build_gen<NC, PT>(prm, e, mc, kind, kidx, G, tc, l1, sc, a.nan_l1, m, s, isn);
-          if (s == 0) {
-            taylor<NC, PT>(G, tc, kind, kidx, dwp, m, M, 1.0, dM, M, dM);  // in place
-            if (isn) {
-#pragma unroll
-              for (int i = 0; i < N; i++) {
-                M[i] = __builtin_nan("");
-#pragma unroll
-                for (int p = 0; p < PTA; p++) dM[p][i] = __builtin_nan("");
-              }
-            }
-          } else {
-            double R[N], dR[PTA][N];
-            event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
-            apply_cols<NC, PT, TEAM>(R, dR, M, dM, lane);
-          }
-        }
-        if (e.type == BMC_EV_GRADIENT || e.type == BMC_EV_SPOILER) {  // simulate.py:55-72
-#pragma unroll
-          for (int i = 0; i < 2 * NT; i++) {
-            M[i] = 0.0;
-#pragma unroll
-            for (int p = 0; p < PTA; p++) dM[p][i] = 0.0;
-          }
-        }
-        if (a.traj && !dead && q == 0 && lane == 0) {  // the trajectory: last_M after every event
-          const long base = (((long)v * a.O + o) * a.T + ei) * N;
-#pragma unroll
-          for (int i = 0; i < N; i++) store_real(a.traj, base + i, M[i], a.dtype);
-        }
-        if (e.type == BMC_EV_ADC && a.reset_after_adc) pending = true;  // simulate.py:105-114
-      }
-    } else {
-      if (pending) {
-        load_start();
-        pending = false;
-      }
-      // the state is not touched while the block is chained: park it in LDS
-      double* st = sh_state[tib];
-      if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-          st[i] = M[i];
-#pragma unroll
-          for (int p = 0; p < PTA; p++) st[(1 + p) * N + i] = dM[p][i];
-        }
-      }
-      double Pi[N], dPi[PTA][N];
-#pragma unroll
-      for (int i = 0; i < N; i++) {
-        Pi[i] = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-        for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
-      }
-      Ev e_next = evr[op.e0];
-      for (int ei = op.e0; ei < op.e0 + op.len; ei++) {
-        const Ev e = e_next;
-        if (ei + 1 < op.e0 + op.len) e_next = evr[ei + 1];  // the next record is in flight meanwhile
-        if (e.dt > 1e-9) {
-          Gen<NC> G;
-          TanCo tc[PTA];
-          double l1, sc;
-          int m, s;
-          bool isn;
-          build_gen<NC, PT>(prm, e, mc, kind, kidx, G, tc, l1, sc, a.nan_l1, m, s, isn);
-          if (s == 0) {
-            // exp(X) Pi = T_m(X) Pi: each lane runs the series on its own column of the chained
-            // propagator (homogeneous coordinate Pi[N][j] = [j == N]); no dense product is needed
-            taylor<NC, PT>(G, tc, kind, kidx, dwp, m, Pi, (lane == N) ? 1.0 : 0.0, dPi, Pi, dPi);  // in place
-            if (isn) {
-#pragma unroll
-              for (int i = 0; i < N; i++) {
-                Pi[i] = __builtin_nan("");
-#pragma unroll
-                for (int p = 0; p < PTA; p++) dPi[p][i] = __builtin_nan("");
-              }
-            }
-          } else {
-            double R[N], dR[PTA][N];
-            event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
-            mul_cols<NC, PT, TEAM>(R, dR, Pi, dPi, lane);
-          }
-        }
-        if (e.type == BMC_EV_GRADIENT || e.type == BMC_EV_SPOILER) {
-#pragma unroll
-          for (int i = 0; i < 2 * NT; i++) {
-            Pi[i] = 0.0;
-#pragma unroll
-            for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
-          }
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < N; i++) {
-        M[i] = st[i];
-#pragma unroll
-        for (int p = 0; p < PTA; p++) dM[p][i] = st[(1 + p) * N + i];
-      }
-      for (int c = 0; c < op.count; c++) apply_cols<NC, PT, TEAM>(Pi, dPi, M, dM, lane);
-    }
-  }
-
+  run_row<NC, PT, TEAM>(prm, st0, sh_state[tib], kind, kidx, dwp, a.ev + (long)o * a.T, a.ops, a.row_ptr[o],
+                        a.row_ptr[o + 1], a.reset_after_adc, a.nan_l1, lane, M, dM,
+                        (a.traj && !dead && q == 0) ? a.traj : nullptr, ((long)v * a.O + o) * a.T * N, a.dtype);
   if (dead || lane != 0) return;
   const long vo = (long)v * a.O + o;
   if (q == 0) {

@@ -172,6 +172,39 @@ int bmc_simulate_adjoint(const bmc_plan* plan, int n_voxels, int n_cest, int dty
                          void* workspace_dev, int64_t workspace_bytes, void* stream);
 
 /*
+ * Uneven batched LM: independent fit problems of different sizes (CEST pool count 1..4, number of rows,
+ * number of free values) fitted together. Problem i owns rows [row0[i], row0[i] + n_rows[i]) of the plan
+ * (its rungs x offsets, front-padded to the plan's event count) and n_params[i] consecutive entries of
+ * the per-value tables (slots, scales, bounds in scaled units, x). params_dev[c] / m0_dev[c] hold the
+ * packed records [n_c, 13 + 5 (c + 1)] and start states of the problems with c + 1 CEST pools, in the
+ * order they appear. Every simulate sweep is one persistent launch over (problem, row, tangent pass)
+ * items; every iteration is that sweep, the float64 normal equations and damped solve per problem, a
+ * primal sweep at the trial values and the accept/reject step, as bmc_lm_fit.
+ *
+ * Replaces: one `fit` per problem (fitting/independent.py:226-298 over lm.py:20-67); vmap cannot batch
+ * problems of different shapes.
+ *
+ * bmc_ubatch_configure sets the launch mapping: tangents per pass (1..2), persistent grid size in
+ * one-wave workgroups, item order (0 problem order, 1 largest first). The result does not depend on
+ * it. bmc_ubatch_simulate: z_out [R] float64 and, if jac_out is not NULL, the ragged Jacobian
+ * [sum n_rows[i] n_params[i]] float64 at x [sum n_params] (scaled). bmc_ubatch_lm_fit: x in/out;
+ * z_meas [R] in dtype; cost_dev [n] float64 (final cost) and n_iter_done_dev [n] optional; flags as
+ * bmc_lm_fit.
+ */
+typedef struct bmc_ubatch bmc_ubatch;
+int bmc_ubatch_create(const bmc_plan* plan, int n_problems, int dtype, const int32_t* n_cest, const int32_t* n_params,
+                      const int32_t* row0, const int32_t* n_rows, const int32_t* slots, const double* scales,
+                      const double* lo, const double* hi, const void* const* params_dev, const void* const* m0_dev,
+                      bmc_ubatch** out);
+int bmc_ubatch_configure(bmc_ubatch* batch, int tangents_per_pass, int grid, int order);
+int bmc_ubatch_sizes(const bmc_ubatch* batch, int* n_rows, int* n_values, int64_t* n_jacobian);
+int bmc_ubatch_simulate(bmc_ubatch* batch, const double* x_dev, double* z_out_dev, double* jac_out_dev, void* stream);
+int bmc_ubatch_lm_fit(bmc_ubatch* batch, const void* z_meas_dev, double* x_dev, int n_iter, double lam0,
+                      unsigned flags, double early_ftol, double early_rms, double* cost_dev, int32_t* n_iter_done_dev,
+                      void* stream);
+int bmc_ubatch_destroy(bmc_ubatch* batch);
+
+/*
  * The learned LM proposal: an MLP trained and evaluated with hand-written BF16 MFMA GEMMs (FP32
  * accumulate), GELU hidden layers, a linear output, mean-squared-error loss and Adam. No counterpart
  * in the reference; its output is the LM start x0 (plain x0 remains the fallback).
