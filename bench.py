@@ -174,6 +174,146 @@ def bench_design(args):
     print(json.dumps(out), flush=True)
 
 
+def uneven_workload(n_problems=4096, seed=3, tsat=2.0):
+    """Heterogeneous QUESP fit problems: 1-8 rungs from {0.5, 1, 2, 3, 4, 5, 6, 8} uT (2 s CW), 31-121 offsets
+    per rung, water plus 1-4 CEST pools at 1-4 ppm and an optional MT pool, 2-8 free values, seeded."""
+    from bmc_jax.fitting import FitSpec, FreeParam
+    from bmc_jax.fitting.uneven import UnevenProblem
+    from bmc_jax.sequences import cw_saturation
+    from bmc_jax.simulation.events import concat_events
+    from bmc_jax.simulation.params import SimParams
+    rng = np.random.default_rng(seed)
+    probs, truths = [], []
+    for _ in range(n_problems):
+        nc = int(rng.integers(1, 5))
+        mt = bool(rng.integers(0, 2))
+        n_rungs = int(rng.integers(1, 9))
+        n_off = int(rng.integers(31, 122))
+        k = np.exp(rng.uniform(np.log(20.0), np.log(3000.0), nc))
+        f = rng.uniform(5e-4, 1e-2, nc)
+        dw = rng.uniform(1.0, 4.0, nc)
+        t1w, t2w, b0i = rng.uniform(0.9, 2.5), rng.uniform(0.04, 0.12), rng.uniform(-0.2, 0.2)
+        a = lambda v: np.asarray(v, np.float64)
+        sp = SimParams(water_r1=a(1 / t1w), water_r2=a(1 / t2w), water_f=a(1.0), cest_k=a(k), cest_f=a(f),
+                       cest_dw=a(dw), cest_r1=a(np.full(nc, 1 / 1.3)), cest_r2=a(np.full(nc, 20.0)),
+                       mt_k=a(20.0 if mt else 0.0), mt_f=a(0.05 if mt else 0.0), mt_dw=a(-2.5 if mt else 0.0),
+                       mt_r1=a(1.0 if mt else 0.0), mt_r2=a(1e5 if mt else 1.0), mt_lineshape=np.int32(1 if mt else 0),
+                       b0=a(3.0), gamma=a(267.5153), b0_inhom=a(b0i), rel_b1=a(1.0))
+        m0 = np.zeros(3 * (1 + nc) + 1)
+        m0[2 * (1 + nc)] = 1.0
+        m0[2 * (1 + nc) + 1:3 * (1 + nc)] = f
+        m0[-1] = 0.05 if mt else 0.0
+        b1s = rng.choice([0.5, 1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 8.0], size=n_rungs, replace=False)
+        ev = concat_events([cw_saturation(np.linspace(-6, 6, n_off), b1_uT=float(b), tsat_s=tsat, b0_T=3.0)
+                            for b in b1s])
+        cands = []
+        for i in range(nc):
+            cands += [FreeParam(f"k{i}", "cest_k", i, 100.0, 1.0, 1e4, k[i]),
+                      FreeParam(f"f{i}", "cest_f", i, 0.001, 1e-5, 0.05, f[i])]
+        cands += [FreeParam("r1w", "water_r1", None, 1.0, 0.1, 3.0, 1 / t1w),
+                  FreeParam("r2w", "water_r2", None, 10.0, 1.0, 60.0, 1 / t2w),
+                  FreeParam("db0", "b0_inhom", None, 0.1, -0.5, 0.5, b0i)]
+        if mt:
+            cands += [FreeParam("kmt", "mt_k", None, 10.0, 1.0, 100.0, 20.0),
+                      FreeParam("fmt", "mt_f", None, 0.01, 1e-3, 0.3, 0.05)]
+        P = int(min(rng.integers(2, 9), len(cands)))
+        spec = FitSpec([cands[j] for j in sorted(rng.choice(len(cands), size=P, replace=False))])
+        truth = spec.x0_phys.copy()
+        x0 = np.clip(truth * np.exp(rng.normal(0.0, 0.2, P)), spec.lo_phys, spec.hi_phys)
+        x0 = np.where(np.abs(truth) < 1e-12, truth + 0.02, x0)
+        probs.append(UnevenProblem(params=sp, events=ev, m0=m0, spec=spec, z_meas=np.zeros(ev.dt.shape[0]),
+                                   x0_phys=x0))
+        truths.append(truth)
+    return probs, truths
+
+
+def bench_uneven(args):
+    import torch
+    from bmc_jax import _flops, _native
+    from bmc_jax.fitting.independent import _lm_batched
+    from bmc_jax.fitting.uneven import UnevenBatch, autotune
+    from bmc_jax.simulation.params import n_cest_of, pack
+    from bmc_jax.simulation.voxels import tile_params
+    from bmc_jax.fitting.residual import spec_slots
+    t_build = time.perf_counter()
+    probs, truths = uneven_workload(args.problems)
+    b = UnevenBatch(probs, dtype=np.float32)
+    # measured Z: every problem at its truth (the batch's own sweep), plus sigma 1e-3 Gaussian noise
+    x_true = np.concatenate([p.spec.to_scaled(t) for p, t in zip(probs, truths)])
+    z, _ = b.simulate(x_true, jacobian=False)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(9)
+    b.z_meas = (z + 1e-3 * torch.randn(z.shape, generator=gen, device="cuda", dtype=torch.float64)).to(torch.float32)
+    t_build = time.perf_counter() - t_build
+    best, table = autotune(b, repeats=2, log=lambda m: print(m, file=sys.stderr, flush=True))
+    n_iter = 30
+    for _ in range(args.warmup):
+        b.lm_fit(n_iter=n_iter)
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, cost, nd = b.lm_fit(n_iter=n_iter)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    wall = float(np.mean(times))
+    rms = np.sqrt(cost / b.n_rows)
+    # the tangent sweep alone, and its flops from a sample of problems
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    xs = b.x0_scaled()
+    b.simulate(xs)
+    e0.record()
+    b.simulate(xs)
+    e1.record()
+    torch.cuda.synchronize()
+    sweep_ms = e0.elapsed_time(e1)
+    ops, rp = b.plan.program()
+    ev_all = {k: np.asarray(getattr(b.plan_events, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    rng = np.random.default_rng(0)
+    sample = rng.choice(b.n, size=min(64, b.n), replace=False)
+    fl = 0.0
+    for i in sample:
+        p = probs[i]
+        nc = n_cest_of(p.params)
+        r0, nr = int(b.row0[i]), int(b.n_rows[i])
+        ev_i = {k: v[r0:r0 + nr] for k, v in ev_all.items()}
+        from bmc_jax.fitting.params import write_params
+        pw = write_params(p.params, p.spec.to_scaled(p.x0_phys), p.spec)
+        fl += float(_flops.executed_flops(pack(pw, 1), nc, ev_i, ops, rp[r0:r0 + nr + 1], spec_slots(p.spec, nc),
+                                          pt=best[0])[0])
+    flops_sweep = fl / len(sample) * b.n
+    achieved = flops_sweep / (sweep_ms * 1e-3) / 1e12
+    # one bmc_lm_fit per problem (what a per-shape loop of `fit` calls launches), on a sample
+    n_one = min(64, b.n)
+    zm = b.z_meas.cpu().numpy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_one):
+        p = probs[i]
+        nc = n_cest_of(p.params)
+        r0, nr = int(b.row0[i]), int(b.n_rows[i])
+        _lm_batched(tile_params(p.params, 1), p.events, np.asarray(p.m0)[None], 2 * (1 + nc), p.spec,
+                    zm[None, r0:r0 + nr], p.spec.to_scaled(p.x0_phys)[None], n_iter, np.float32)
+    torch.cuda.synchronize()
+    per_problem_s = (time.perf_counter() - t0) / n_one
+    out = {"metric": "LM-fitted problems/s (30 iterations, uneven batch)", "value": b.n / wall,
+           "unit": "problems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 (float32 I/O)",
+           "data": "synthetic (seeded problems, sigma 1e-3 noise)",
+           "config": {"workload": "uneven batched LM: 4096 QUESP problems, 1-8 rungs x 31-121 offsets (2 s CW), "
+                                  "1-4 CEST pools + optional MT, 2-8 free values, 30 LM iterations in one hipGraph",
+                      "problems": b.n, "rows": int(b.R), "values": int(b.n_values),
+                      "autotuned": {"tangents_per_pass": best[0], "grid": best[1], "order": best[2]}},
+           "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (persistent sweep)",
+                        "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "flops_per_launch": flops_sweep,
+                        "avg_launch_ms": sweep_ms},
+           "autotune_ms": {f"{k[0]},{k[1]},{k[2]}": v for k, v in table.items()},
+           "per_problem_lm_fit_s": per_problem_s, "speedup_vs_per_problem_launches": per_problem_s * b.n / wall,
+           "median_rms": float(np.median(rms)), "setup_s": t_build}
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(prm, m0, ev, nc, slots, budget_s=15.0):
     """The C oracle on this host's cores: whole spectra of the first voxels, as many as fit the budget."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -203,7 +343,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flop-sample", type=int, default=16)
     ap.add_argument("--truths", type=int, default=1024)
-    ap.add_argument("--workload", choices=("pulsed", "nn", "design"), default="pulsed",
+    ap.add_argument("--problems", type=int, default=4096)
+    ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven"), default="pulsed",
                     help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
     ap.add_argument("--nn-train", type=int, default=2_000_000)
     ap.add_argument("--nn-test", type=int, default=262_144)
@@ -218,6 +359,10 @@ def main():
     from bmc_jax import _engine, _flops, _native
     from bmc_jax.simulation.events import SimEvent
 
+    if args.workload == "uneven":
+        _native.torch_cuda()
+        bench_uneven(args)
+        return
     if args.workload == "design":
         _native.torch_cuda()
         bench_design(args)

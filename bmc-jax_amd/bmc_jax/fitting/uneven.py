@@ -106,7 +106,8 @@ class UnevenBatch:
         self.row0, self.n_rows, self.n_params = np.array(row0), np.array(n_rows), np.array(nps)
         self.x_off = np.concatenate([[0], np.cumsum(nps)])
         self.scales = np.array(scales)
-        self.plan = _engine.plan_for(concat_events(rows))
+        self.plan_events = concat_events(rows)
+        self.plan = _engine.plan_for(self.plan_events)
         self._keep = []
         prm_ptrs = (ctypes.c_void_p * 4)()
         m0_ptrs = (ctypes.c_void_p * 4)()
