@@ -28,6 +28,10 @@ sys.path[:0] = [os.path.join(ROOT, "bmc-jax_amd")]
 
 FREE = [("cest_k", 0), ("cest_f", 0), ("water_r1", None), ("water_r2", None), ("b0_inhom", None)]
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes per voxel of one launch of this workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 1.146e12 B
+# per 65,536-voxel launch, almost all register-spill traffic.
+TRAFFIC_BYTES_PER_VOXEL = 1.146e12 / 65536
 
 
 def workload(n_vox, seed=2):
@@ -420,7 +424,8 @@ def main():
     rng = np.random.default_rng(0)
     sample = rng.choice(V, size=min(args.flop_sample, V), replace=False)
     ev_np = {k: np.asarray(getattr(ev, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
-    per_vox = _flops.executed_flops(prm[sample].astype(np.float64), nc, ev_np, ops, rp, slots, pt=1)
+    pt = int(_native.lib().bmc_tangents_per_pass(len(slots)))
+    per_vox = _flops.executed_flops(prm[sample].astype(np.float64), nc, ev_np, ops, rp, slots, pt=pt)
     flops_launch = float(per_vox.mean()) * V
     avg_launch_s = 1e-3 * float(np.mean(launch_ms))
     achieved = flops_launch / avg_launch_s / 1e12
@@ -447,7 +452,8 @@ def main():
                    "io_dtype": "float32", "plan_built_events_per_offset": stats["n_built"] / 61},
         "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (MI355X's FP64 vector "
                      "and FP64 matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES_PER_VOXEL * V,
+                     "traffic_unit": "bytes per launch (measured, scaled by voxels)",
                      "flops_per_launch": flops_launch, "avg_launch_ms": 1e3 * avg_launch_s},
     }
     if not args.no_cpu:

@@ -72,6 +72,8 @@ def lib():
         L.bmc_lm_fit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ip32, dp, dp, dp,
                                  ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_uint,
                                  ctypes.c_double, ctypes.c_double, vp, vp, vp, vp]
+        L.bmc_tangents_per_pass.argtypes = [ctypes.c_int]
+        L.bmc_tangents_per_pass.restype = ctypes.c_int
         L.bmc_adjoint_workspace_size.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, i64p]
         L.bmc_simulate_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32,
                                            dp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int64, vp]

@@ -33,7 +33,7 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
+def build(verbose: bool = False, jobs: int = 4, defines=()) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     objs_dir = os.path.join(HERE, "_build")
     os.makedirs(objs_dir, exist_ok=True)
@@ -42,7 +42,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     for src in SOURCES:
         obj = os.path.join(objs_dir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [hipcc()] + HIPCC_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc()] + HIPCC_FLAGS + [f"-D{d}" for d in defines] + inc + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

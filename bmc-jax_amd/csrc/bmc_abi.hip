@@ -275,9 +275,14 @@ int fill_sim_args(const bmc_plan* plan, int V, int nc, int dtype, const void* pa
 }
 
 // Tangents per pass for a tangent count (register budget of the kernel).
-int tangents_per_pass(int P) { return P <= 0 ? 0 : 1; }
+#ifndef BMC_TANGENTS_PER_PASS
+#define BMC_TANGENTS_PER_PASS 2
+#endif
+int tangents_per_pass(int P) { return P <= 0 ? 0 : (P < BMC_TANGENTS_PER_PASS ? P : BMC_TANGENTS_PER_PASS); }
 
 }  // namespace bmc
+
+extern "C" int bmc_tangents_per_pass(int n_tangents) { return tangents_per_pass(n_tangents); }
 
 extern "C" int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
                             const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,

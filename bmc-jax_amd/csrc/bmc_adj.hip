@@ -203,8 +203,12 @@ __device__ __forceinline__ TanCo rf_tangent(const double* prm, const Ev& e, cons
   return t;
 }
 
+#ifndef BMC_ADJ_WAVES_PER_SIMD
+#define BMC_ADJ_WAVES_PER_SIMD 1
+#endif
+
 template <int NC, int P, int TEAM>
-__global__ void __launch_bounds__(256) adj_kernel(AdjArgs a) {
+__global__ void __launch_bounds__(256, BMC_ADJ_WAVES_PER_SIMD) adj_kernel(AdjArgs a) {
   constexpr int NT = 1 + NC, N = 3 * NT + 1;
   constexpr int TPB = 256 / TEAM;
   constexpr int PA = P > 0 ? P : 1;

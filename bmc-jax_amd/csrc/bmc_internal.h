@@ -93,6 +93,8 @@ struct Plan {
 void set_error(const std::string& msg);
 int tan_kind_of_slot(int slot, int nc, int* idx);
 int launch_sim(int nc, const SimArgs& a, hipStream_t stream);
+// Tangents carried per simulate pass for P tangents (register budget of the kernel).
+int tangents_per_pass(int P);
 
 }  // namespace bmc
 

@@ -282,8 +282,8 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   sj.jac_out = bJ.p;
   sj.jac_dtype = BMC_F64;
   sj.active = st.active;
-  sj.PT = 1;
-  sj.n_pass = P;
+  sj.PT = tangents_per_pass(P);
+  sj.n_pass = (P + sj.PT - 1) / sj.PT;
   // launch 3: primal at the trial point
   SimArgs sp = sj;
   sp.xw = st.xt;

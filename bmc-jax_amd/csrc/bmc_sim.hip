@@ -28,8 +28,13 @@
 
 namespace bmc {
 
+// Waves per SIMD the register allocation is held to (2 halves the registers per lane).
+#ifndef BMC_SIM_WAVES_PER_SIMD
+#define BMC_SIM_WAVES_PER_SIMD 2
+#endif
+
 template <int NC, int PT, int TEAM>
-__global__ void __launch_bounds__(256) sim_kernel(SimArgs a) {
+__global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArgs a) {
   constexpr int NT = 1 + NC, N = 3 * NT + 1;
   constexpr int TPB = 256 / TEAM;
   using I = PrmIdx<NC>;

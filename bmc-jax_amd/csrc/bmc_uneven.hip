@@ -135,8 +135,12 @@ __device__ __forceinline__ void u_chunk(const USimArgs& a, int2 ch, double (*sh_
   }
 }
 
+#ifndef BMC_USIM_WAVES_PER_SIMD
+#define BMC_USIM_WAVES_PER_SIMD 1
+#endif
+
 template <int PT>
-__global__ void __launch_bounds__(64) usim_kernel(USimArgs a) {
+__global__ void __launch_bounds__(64, BMC_USIM_WAVES_PER_SIMD) usim_kernel(USimArgs a) {
   constexpr int PTA = PT > 0 ? PT : 1;
   static_assert(PTA <= 3, "tangents per pass");
   __shared__ double sh_prm[4][PrmIdx<4>::size];

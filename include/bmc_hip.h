@@ -56,6 +56,9 @@ extern "C" {
 
 int bmc_abi_version(void);
 const char* bmc_last_error(void);
+/* Tangents bmc_simulate and bmc_lm_fit carry per pass of the kernel for n_tangents tangents (the
+ * kernel runs ceil(n_tangents / that) passes, each with the primal state). Informational. */
+int bmc_tangents_per_pass(int n_tangents);
 
 /* Number of HIP devices visible (0 if none). */
 int bmc_device_count(void);
