@@ -383,7 +383,8 @@ def main():
                "data": "synthetic (simulator-generated, seeded)",
                "config": {"workload": "learned LM proposal: 2-pool CW QUESP ladder (0.5, 1, 2, 4 uT, 2 s), 61 offsets, "
                                       "MLP 244-512-512-256-5 GELU", "n_train": args.nn_train, "n_test": args.nn_test},
-               "plain_lm": r["plain"], "nn_lm": r["nn"], "speedup_vs_plain": r["speedup"],
+               "plain_lm_early_exit": r["plain"], "plain_lm_all_iterations": r["plain_fixed"], "nn_lm": r["nn"],
+               "speedup_vs_plain_early_exit": r["speedup"], "speedup_vs_plain_all_iterations": r["speedup_vs_fixed"],
                "train_seconds": r["train_seconds"]}
         print(json.dumps(out), flush=True)
         return

@@ -204,12 +204,13 @@ def simulate_trajectory(plan: Plan, params, n_cest: int, dtype_code: int, m0):
 
 
 def lm_fit(plan: Plan, params, n_cest: int, dtype_code: int, m0, slots, scales, lo, hi, jc, z_meas, x0, n_iter,
-           lam0=1e-2, free=None, flags=0, early_ftol=0.0, early_rms=0.0, want_traj=False):
-    """bmc_lm_fit on device tensors. x0 [V, P] float64 (scaled). Returns (x, cost_traj, x_traj, n_done)."""
+           lam0=1e-2, free=None, flags=0, early_ftol=0.0, early_rms=0.0, want_traj=False, want_cost=False):
+    """bmc_lm_fit on device tensors. x0 [V, P] float64 (scaled). Returns (x, cost_traj, x_traj, n_done);
+    cost_traj [V, n_iter] comes with want_traj or want_cost, x_traj only with want_traj."""
     torch = torch_cuda()
     V, P = int(x0.shape[0]), len(slots)
     x = x0.to(device="cuda", dtype=torch.float64).contiguous().clone()
-    ctraj = torch.empty((V, n_iter), dtype=torch.float64, device="cuda") if want_traj else None
+    ctraj = torch.empty((V, n_iter), dtype=torch.float64, device="cuda") if (want_traj or want_cost) else None
     xtraj = torch.empty((V, n_iter, P), dtype=torch.float64, device="cuda") if want_traj else None
     ndone = torch.empty((V,), dtype=torch.int32, device="cuda")
     arr = lambda ct, vals: (ct * max(len(vals), 1))(*vals)

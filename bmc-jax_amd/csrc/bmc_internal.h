@@ -77,6 +77,8 @@ struct SimArgs {
   double nan_l1;          // reference NaN threshold on the augmented 1-norm
   void* traj;             // [V, O, T, N] dtype or null: state after every event (SEQ-only plans)
   long team0;             // first team of this launch (a launch covers at most 2^31 threads)
+  const int32_t* vlist;   // [<= V] or null: simulate only these voxels (compacted list)
+  const int32_t* n_vlist; // device count of vlist
 };
 
 struct Plan {

@@ -32,6 +32,7 @@ struct LmState {
   double* lam;      // [V]
   double* cost;     // [V] cost at x (set by normal_eq)
   int32_t* active;  // [V]
+  int32_t* jac_live;  // [V] active and moved by the last step: its J and z must be recomputed
   int32_t* it;      // [1] iteration counter
   int32_t* n_done;  // [V] or null
   double* ctraj;    // [V, n_iter] or null
@@ -177,6 +178,8 @@ __global__ void __launch_bounds__(64) accept_kernel(AcArgs a) {
   if (lane != 0) return;
   const double lam = a.s.lam[v];
   a.s.lam[v] = better ? lam * 0.5 : lam * 3.0;  // lm.py:59-60
+  // a rejected step leaves x, hence z and J, unchanged: the next iteration reuses them
+  a.s.jac_live[v] = better ? 1 : 0;
   const double cout = better ? c : c0;
   if (better) a.s.cost[v] = c;
   if (a.s.ctraj) a.s.ctraj[(long)v * a.n_iter + it] = cout;
@@ -185,8 +188,28 @@ __global__ void __launch_bounds__(64) accept_kernel(AcArgs a) {
     const bool small = sqrt(cout / (double)a.O) <= a.rms_tol;
     const bool flat = better && (c0 - c) <= a.ftol * c0;
     const bool stuck = a.s.lam[v] > 1e12;
-    if (small || flat || stuck) a.s.active[v] = 0;
+    if (small || flat || stuck) {
+      a.s.active[v] = 0;
+      a.s.jac_live[v] = 0;
+    }
   }
+}
+
+// list <- indices v with flag[v] != 0 (any order), count <- their number (count cleared beforehand).
+// One atomic per wave: the wave's hits are ranked by ballot.
+__global__ void __launch_bounds__(256) compact_kernel(const int32_t* flag, int V, int32_t* list, int32_t* count) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool f = v < V && flag[v] != 0;
+  const unsigned long long m = __ballot(f);
+  const int lane = threadIdx.x % 64;
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, 0, 64);
+  if (f) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+__global__ void clear2_kernel(int32_t* c) {
+  if (threadIdx.x < 2 && blockIdx.x == 0) c[threadIdx.x] = 0;
 }
 
 __global__ void bump_iter_kernel(int32_t* it) {
@@ -199,6 +222,7 @@ __global__ void init_kernel(LmState s, int V, double lam0) {
   if (v < V) {
     s.lam[v] = lam0;
     s.active[v] = 1;
+    s.jac_live[v] = 1;
     s.cost[v] = 0.0;
     if (s.n_done) s.n_done[v] = 0;
   }
@@ -246,7 +270,7 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
     }
   } sg{own};
 
-  DevBuf bz, bJ, bzt, bxt, blam, bcost, bact, bit;
+  DevBuf bz, bJ, bzt, bxt, blam, bcost, bact, bjl, bit, blj, bla, bcnt;
   auto alloc = [&](DevBuf& b, size_t n) -> int {
     if (hipMalloc(&b.p, n > 0 ? n : 8) != hipSuccess) {
       set_error("bmc_lm_fit: device allocation of " + std::to_string(n) + " bytes failed");
@@ -257,7 +281,9 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   if ((rc = alloc(bz, sizeof(double) * V * O)) || (rc = alloc(bJ, sizeof(double) * (size_t)V * O * P)) ||
       (rc = alloc(bzt, sizeof(double) * V * O)) || (rc = alloc(bxt, sizeof(double) * V * P)) ||
       (rc = alloc(blam, sizeof(double) * V)) || (rc = alloc(bcost, sizeof(double) * V)) ||
-      (rc = alloc(bact, sizeof(int32_t) * V)) || (rc = alloc(bit, sizeof(int32_t))))
+      (rc = alloc(bact, sizeof(int32_t) * V)) || (rc = alloc(bjl, sizeof(int32_t) * V)) ||
+      (rc = alloc(bit, sizeof(int32_t))) || (rc = alloc(blj, sizeof(int32_t) * V)) ||
+      (rc = alloc(bla, sizeof(int32_t) * V)) || (rc = alloc(bcnt, sizeof(int32_t) * 2)))
     return rc;
 
   LmState st;
@@ -266,6 +292,7 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   st.lam = (double*)blam.p;
   st.cost = (double*)bcost.p;
   st.active = (int32_t*)bact.p;
+  st.jac_live = (int32_t*)bjl.p;
   st.it = (int32_t*)bit.p;
   st.n_done = n_iter_done_dev;
   st.ctraj = cost_traj_dev;
@@ -281,11 +308,17 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   sj.z_out = (double*)bz.p;
   sj.jac_out = bJ.p;
   sj.jac_dtype = BMC_F64;
-  sj.active = st.active;
+  // each sweep runs over a compacted list of the voxels it has to simulate
+  int32_t* cnt = (int32_t*)bcnt.p;
+  sj.active = nullptr;
+  sj.vlist = (const int32_t*)blj.p;
+  sj.n_vlist = cnt;
   sj.PT = tangents_per_pass(P);
   sj.n_pass = (P + sj.PT - 1) / sj.PT;
   // launch 3: primal at the trial point
   SimArgs sp = sj;
+  sp.vlist = (const int32_t*)bla.p;
+  sp.n_vlist = cnt + 1;
   sp.xw = st.xt;
   sp.z_out = (double*)bzt.p;
   sp.jac_out = nullptr;
@@ -327,8 +360,15 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
 
   auto iteration = [&]() -> int {
     int r;
+    hipLaunchKernelGGL(clear2_kernel, dim3(1), dim3(64), 0, stream, cnt);
+    hipLaunchKernelGGL(compact_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, (const int32_t*)st.jac_live, V,
+                       (int32_t*)blj.p, cnt);
+    BMC_HIP_CHECK(hipGetLastError());
     if ((r = launch_sim(n_cest, sj, stream)) != BMC_OK) return r;
     hipLaunchKernelGGL(normal_eq_kernel, dim3(V), dim3(64), 0, stream, ne);
+    BMC_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(compact_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, (const int32_t*)st.active, V,
+                       (int32_t*)bla.p, cnt + 1);
     BMC_HIP_CHECK(hipGetLastError());
     if ((r = launch_sim(n_cest, sp, stream)) != BMC_OK) return r;
     hipLaunchKernelGGL(accept_kernel, dim3(V), dim3(64), 0, stream, ac);

@@ -33,28 +33,30 @@ namespace bmc {
 #define BMC_SIM_WAVES_PER_SIMD 2
 #endif
 
-template <int NC, int PT, int TEAM>
-__global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArgs a) {
+// One team's (voxel, offset row, pass) of a launch with `total` teams over n_vox voxels (voxel
+// index through a.vlist when a compacted list is given). Every thread of the block calls it: it
+// holds block barriers before its first early exit.
+template <int NC, int PT, int TEAM, bool LISTED>
+__device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total, int n_vox,
+                                         double (*sh_prm)[PrmIdx<NC>::size],
+                                         double (*sh_state)[(1 + (PT > 0 ? PT : 1)) * (3 * (1 + NC) + 1)],
+                                         double (*sh_start)[(1 + (PT > 0 ? PT : 1)) * (3 * (1 + NC) + 1)],
+                                         int* sh_tk, int* sh_ti) {
   constexpr int NT = 1 + NC, N = 3 * NT + 1;
-  constexpr int TPB = 256 / TEAM;
   using I = PrmIdx<NC>;
   constexpr int PTA = PT > 0 ? PT : 1;
-  __shared__ double sh_prm[TPB][I::size];
-  __shared__ double sh_state[TPB][(1 + PTA) * N];  // the state, parked while a block is chained
-  __shared__ double sh_start[TPB][(1 + PTA) * N];  // the start state and its tangent seeds
-  __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
-
   const int lane = threadIdx.x % TEAM;
   const int tib = threadIdx.x / TEAM;
-  const long team = a.team0 + (long)blockIdx.x * TPB + tib;
-  const long total = (long)a.V * a.O * a.n_pass;
   bool dead = team >= total;
   const long tt = dead ? 0 : team;
   const int o = (int)(tt % a.O);
   const long t2 = tt / a.O;
-  const int v = (int)(t2 % a.V);
-  const int q = (int)(t2 / a.V);
+  const int vi = (int)(t2 % n_vox);
+  const int v = LISTED ? a.vlist[vi] : vi;
+  const int q = (int)(t2 / n_vox);
   if (!dead && a.active && !a.active[v]) dead = true;
+  // a block with no live team leaves before touching its records (stopped LM problems)
+  if (a.active && !__syncthreads_or(!dead)) return;
 
   if (threadIdx.x < kMaxTan) {
     sh_tk[threadIdx.x] = threadIdx.x < a.P ? a.t_kind[threadIdx.x] : TK_NONE;
@@ -140,12 +142,53 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArg
   }
 }
 
+template <int NC, int PT, int TEAM>
+__global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArgs a) {
+  constexpr int N = 3 * (1 + NC) + 1;
+  constexpr int TPB = 256 / TEAM;
+  constexpr int PTA = PT > 0 ? PT : 1;
+  __shared__ double sh_prm[TPB][PrmIdx<NC>::size];
+  __shared__ double sh_state[TPB][(1 + PTA) * N];  // the state, parked while a block is chained
+  __shared__ double sh_start[TPB][(1 + PTA) * N];  // the start state and its tangent seeds
+  __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
+  const int tib = threadIdx.x / TEAM;
+  sim_team<NC, PT, TEAM, false>(a, a.team0 + (long)blockIdx.x * TPB + tib, (long)a.V * a.O * a.n_pass, a.V, sh_prm,
+                         sh_state, sh_start, sh_tk, sh_ti);
+}
+
+// The same over a compacted voxel list (a.vlist, device count a.n_vlist): a fixed grid strides over the
+// live teams; the trip count is block-uniform.
+template <int NC, int PT, int TEAM>
+__global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_list_kernel(SimArgs a) {
+  constexpr int N = 3 * (1 + NC) + 1;
+  constexpr int TPB = 256 / TEAM;
+  constexpr int PTA = PT > 0 ? PT : 1;
+  __shared__ double sh_prm[TPB][PrmIdx<NC>::size];
+  __shared__ double sh_state[TPB][(1 + PTA) * N];
+  __shared__ double sh_start[TPB][(1 + PTA) * N];
+  __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
+  const int tib = threadIdx.x / TEAM;
+  const int nv = *a.n_vlist;
+  const long total = (long)nv * a.O * a.n_pass;
+  for (long base = (long)blockIdx.x * TPB; base < total; base += (long)gridDim.x * TPB) {
+    sim_team<NC, PT, TEAM, true>(a, base + tib, total, nv, sh_prm, sh_state, sh_start, sh_tk, sh_ti);
+    __syncthreads();  // the next item overwrites the block's LDS
+  }
+}
+
 template <int NC, int PT>
 static int launch_nc_pt(const SimArgs& a, hipStream_t stream) {
   constexpr int NA = 3 * (1 + NC) + 2;
   constexpr int TEAM = NA <= 16 ? 16 : 32;
   constexpr int TPB = 256 / TEAM;
   const long teams = (long)a.V * a.O * a.n_pass;
+  if (a.vlist) {  // persistent grid over a device-side voxel list of at most V entries
+    const long blocks = std::min<long>((teams + TPB - 1) / TPB, 2048);
+    if (blocks <= 0) return BMC_OK;
+    hipLaunchKernelGGL((sim_list_kernel<NC, PT, TEAM>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    BMC_HIP_CHECK(hipGetLastError());
+    return BMC_OK;
+  }
   // HIP caps a launch at 2^32 - 1 work-items: split the teams into launches of at most 2^23 blocks
   constexpr long kMaxBlocks = 1L << 23;
   for (long t0 = 0; t0 < teams; t0 += kMaxBlocks * TPB) {
