@@ -97,3 +97,70 @@ def test_optimize_design_same_steps_with_either_gradient(setup):
     np.testing.assert_allclose(ra.design, rf.design, rtol=1e-6)
     np.testing.assert_allclose(ra.cost, rf.cost, rtol=1e-8)
     assert ra.cost[-1] < ra.cost[0]
+
+
+# ---- the bound's laws (the reference pins them in tests/test_design.py) --------------------------------
+
+@pytest.fixture(scope="module")
+def crlb_setup():
+    from bmc_jax import _native
+    _native.torch_cuda()
+    from bmc_jax.fitting import Experiment, FitSpec, FreeParam
+    from bmc_jax.io import example_config
+    from bmc_jax.sequences import cw_saturation
+    from bmc_jax.simulation import initial_state, to_sim_params
+    from bmc_jax.simulation.events import concat_events
+    p = example_config("amide")
+    jp = to_sim_params(p)
+    rung = cw_saturation(np.linspace(-6.0, 6.0, 31), b1_uT=1.0, tsat_s=2.0, b0_T=p.system.b0)
+
+    def ladder(b1):
+        ev = concat_events([rung._replace(rf_amp=np.asarray(rung.rf_amp) * b) for b in b1])
+        return Experiment(jp, ev, initial_state(jp), p.mz_loc)
+    K = FreeParam("k", "cest_k", 0, scale=100.0, lo=5.0, hi=5000.0, x0=150.0)
+    F = FreeParam("f", "cest_f", 0, scale=0.001, lo=1e-6, hi=0.05, x0=0.0009)
+    return dict(ladder=ladder, K=K, F=F, FitSpec=FitSpec, FreeParam=FreeParam, Experiment=Experiment)
+
+
+def test_the_bound_scales_with_the_noise(crlb_setup):
+    from bmc_jax.design import crlb
+    c = crlb_setup
+    exp = c["ladder"]((0.5, 1.0, 2.0))
+    a, b = crlb(exp, c["FitSpec"]([c["K"], c["F"]]), 0.002), crlb(exp, c["FitSpec"]([c["K"], c["F"]]), 0.004)
+    np.testing.assert_allclose(b.sd, 2 * a.sd, rtol=1e-5)
+    np.testing.assert_allclose(b.corr, a.corr, atol=1e-6)
+    assert a["k"] == a.sd[0] and a.rel_sd[0] == pytest.approx(a.sd[0] / 150.0)
+
+
+def test_twice_the_data_lowers_the_bound_by_sqrt2(crlb_setup):
+    from bmc_jax.design import crlb
+    c = crlb_setup
+    once = c["ladder"]((0.5, 1.0, 2.0))
+    twice = c["Experiment"].stack([once, once])
+    a, b = crlb(once, c["FitSpec"]([c["K"], c["F"]]), 0.002), crlb(twice, c["FitSpec"]([c["K"], c["F"]]), 0.002)
+    np.testing.assert_allclose(b.sd, a.sd / np.sqrt(2.0), rtol=1e-4)
+
+
+def test_a_parameter_alone_is_bounded_by_its_own_fisher_entry(crlb_setup):
+    from bmc_jax.design import crlb
+    c = crlb_setup
+    exp = c["ladder"]((0.5, 1.0, 2.0))
+    # at the same point (f as the params carry it) the k column is the same column: equal to rounding
+    f_cfg = float(np.asarray(exp.params.cest_f)[0])
+    both = crlb(exp, c["FitSpec"]([c["K"], c["F"]]), 0.002, x=[150.0, f_cfg])
+    alone = crlb(exp, c["FitSpec"]([c["K"]]), 0.002)
+    assert alone.sd[0] == pytest.approx(1.0 / np.sqrt(both.fisher[0, 0]), rel=1e-6)
+    assert alone.sd[0] < both.sd[0]
+
+
+def test_the_bound_needs_one_sample_and_plain_values(crlb_setup):
+    from bmc_jax.design import crlb
+    from bmc_jax.simulation import tile_params
+    c = crlb_setup
+    exp = c["ladder"]((0.5, 1.0, 2.0))
+    with pytest.raises(ValueError, match="one sample"):
+        crlb(c["Experiment"](tile_params(exp.params, 2), exp.events, exp.m0, exp.mz_loc),
+             c["FitSpec"]([c["K"], c["F"]]), 0.002)
+    shared = c["FreeParam"]("r2w", "water_r2", None, scale=1.0, lo=0.5, hi=2.0, x0=1.0, scope="shared", mode="factor")
+    with pytest.raises(ValueError, match="plain values"):
+        crlb(exp, c["FitSpec"]([c["K"], c["F"], shared]), 0.002)
