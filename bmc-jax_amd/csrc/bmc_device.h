@@ -1,5 +1,5 @@
 // Device building blocks of the simulate and adjoint kernels: the sparse generator of one event,
-// its tangents, the truncated Taylor series and the column-layout products of a 16-lane team.
+// its tangents, the truncated Taylor series and the column-layout products of a team of lanes.
 // Included by each HIP translation unit (no relocatable device code: every unit keeps its own copy).
 #pragma once
 
@@ -49,6 +49,11 @@ template <int TEAM, int K>
 __device__ __forceinline__ double bcast(double x) {
   if constexpr (TEAM == 16) {
     return __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);  // row_newbcast:K
+  } else if constexpr (TEAM == 8) {
+    // two teams share a DPP row: the lower half takes lane K, the upper half lane 8 + K
+    const double lo = __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);
+    const double hi = __builtin_amdgcn_mov_dpp(x, 0x150 + 8 + K, 0xf, 0xf, true);
+    return (threadIdx.x & 8) ? hi : lo;
   } else {
     return __shfl(x, K, TEAM);
   }

@@ -2,7 +2,9 @@
 // (src/bmc_jax/simulation/simulate.py:22-148) and its Jacobian (jax.jacfwd at fitting/lm.py:39),
 // written for CDNA4.
 //
-// Work mapping. A team of 16 lanes (one DPP row) simulates one (voxel, offset row, tangent pass).
+// Work mapping. A team of lanes simulates one (voxel, offset row, tangent pass): 8 lanes (two teams per
+// DPP row) when the augmented state has at most 8 entries (one CEST pool), 16 (one DPP row) up to 16
+// entries, 32 beyond.
 //  * The generator X = [[A, C], [0, 0]] dt (matrix.py:83-295, expm.py:98-106) is sparse and
 //    identical on every lane of a team, so each lane rebuilds it from the parameters and applies it
 //    with no communication.
@@ -179,7 +181,7 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_list_kernel(S
 template <int NC, int PT>
 static int launch_nc_pt(const SimArgs& a, hipStream_t stream) {
   constexpr int NA = 3 * (1 + NC) + 2;
-  constexpr int TEAM = NA <= 16 ? 16 : 32;
+  constexpr int TEAM = NA <= 8 ? 8 : (NA <= 16 ? 16 : 32);  // one lane per propagator column
   constexpr int TPB = 256 / TEAM;
   const long teams = (long)a.V * a.O * a.n_pass;
   if (a.vlist) {  // persistent grid over a device-side voxel list of at most V entries
