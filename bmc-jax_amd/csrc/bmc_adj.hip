@@ -5,8 +5,9 @@
 // Given cotangents gz on Z = state[jc] and gj on J_p = dstate[jc]/dx_p * scale_p for every (voxel,
 // row), the kernel accumulates dL/d rf_amp[o, t] and dL/d rf_freq[o, t], summed over voxels.
 //
-// Work mapping. A team of 16 lanes owns one (row, voxel), voxel fastest, so the four teams of a wave
-// usually share a row and their per-event gradients are summed in registers before one atomic add.
+// Work mapping. A team of lanes (8 for at most one CEST pool, else 16) owns one (row, voxel), voxel
+// fastest, so the teams of a wave usually share a row and their per-event gradients are summed in
+// registers before one atomic add.
 //  * Phase 1 runs the scan forward (state M and its P tangents D_p, all on every lane of the team, as
 //    in the simulate kernel) and stores a checkpoint every K = ceil(sqrt(T)) events in the workspace.
 //  * Phase 2 walks the segments backwards. It re-runs a segment forward from its checkpoint, storing
@@ -497,7 +498,8 @@ __global__ void __launch_bounds__(256, BMC_ADJ_WAVES_PER_SIMD) adj_kernel(AdjArg
 
 template <int NC, int P>
 static int launch_adj_np(const AdjArgs& a, hipStream_t stream) {
-  constexpr int TEAM = 16;
+  constexpr int NA = 3 * (1 + NC) + 2;
+  constexpr int TEAM = NA <= 8 ? 8 : 16;  // one lane per propagator column (two teams per DPP row at 8)
   constexpr int TPB = 256 / TEAM;
   const long teams = (long)a.V * a.O;
   constexpr long kMaxBlocks = 1L << 23;
@@ -537,7 +539,7 @@ static long adj_record(int nc, int P) { return (long)(1 + P) * (3 * (1 + nc) + 1
 
 static long adj_teams_padded(int V, int O) {
   const long teams = (long)V * O;
-  return (teams + 15) / 16 * 16;
+  return (teams + 31) / 32 * 32;  // whole blocks of 8- or 16-lane teams
 }
 
 }  // namespace bmc

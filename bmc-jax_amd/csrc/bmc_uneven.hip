@@ -4,8 +4,8 @@
 // (src/bmc_jax/fitting/independent.py:226-298, lm.py:20-67), since vmap needs equal shapes.
 //
 // The simulation is one persistent launch per sweep. Its work items are (problem, row, tangent pass)
-// triples, grouped into chunks of same-class items (one chunk per wave: 4 teams of 16 lanes, or 2
-// teams of 32 for four CEST pools). A fixed grid of one-wave workgroups pulls chunks from a global
+// triples, grouped into chunks of same-class items (one chunk per wave: 8 teams of 8 lanes for one
+// CEST pool, 4 of 16 for two or three, 2 of 32 for four). A fixed grid of one-wave workgroups pulls chunks from a global
 // counter until the list is exhausted, so long and short problems share the machine without a
 // launch per shape. Each team runs the same row program as bmc_simulate (run_row).
 // The LM steps (float64 normal equations, damped solve, clip, accept/reject) are ragged versions of
@@ -143,9 +143,9 @@ template <int PT>
 __global__ void __launch_bounds__(64, BMC_USIM_WAVES_PER_SIMD) usim_kernel(USimArgs a) {
   constexpr int PTA = PT > 0 ? PT : 1;
   static_assert(PTA <= 3, "tangents per pass");
-  __shared__ double sh_prm[4][PrmIdx<4>::size];
-  __shared__ double sh_state[4][(1 + 3) * kUMaxN];
-  __shared__ double sh_start[4][(1 + 3) * kUMaxN];
+  __shared__ double sh_prm[8][PrmIdx<4>::size];  // up to 8 teams per wave (8-lane teams)
+  __shared__ double sh_state[8][(1 + 3) * kUMaxN];
+  __shared__ double sh_start[8][(1 + 3) * kUMaxN];
   __shared__ int sh_chunk;
   for (;;) {
     if (threadIdx.x == 0) sh_chunk = atomicAdd(a.counter, 1);
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(64, BMC_USIM_WAVES_PER_SIMD) usim_kernel(USimA
     const int cls = a.probs[a.items[ch.x].prob].cls;
     switch (cls) {
       case 0:
-        u_chunk<1, PT, 16>(a, ch, sh_prm, sh_state, sh_start);
+        u_chunk<1, PT, 8>(a, ch, sh_prm, sh_state, sh_start);
         break;
       case 1:
         u_chunk<2, PT, 16>(a, ch, sh_prm, sh_state, sh_start);
@@ -386,7 +386,7 @@ void build_items(const bmc_ubatch& b, int PT, int order, bool tangents, std::vec
   items.clear();
   chunks.clear();
   for (int cls = 3; cls >= 0; cls--) {
-    const int per = (cls == 3) ? 2 : 4;
+    const int per = (cls == 3) ? 2 : (cls == 0 ? 8 : 4);  // teams per wave: 32, 16 or 8 lanes
     std::vector<UItem> mine;
     for (int id : ids) {
       const UProb& p = b.h_probs[id];
