@@ -237,7 +237,7 @@ class Proposal:
     train_seconds: float
 
 
-def train_proposal(n_train=2_000_000, epochs=10, batch=4096, lr=1e-3, noise=1e-3, seed=5, log=None) -> Proposal:
+def train_proposal(n_train=2_000_000, epochs=30, batch=4096, lr=1e-3, noise=1e-3, seed=5, log=None) -> Proposal:
     """Simulate n_train spectra on the GPU, add sigma-noise, train the MLP (BF16 MFMA) to the normalised values."""
     torch = _native.torch_cuda()
     rng = np.random.default_rng(seed)
@@ -317,7 +317,7 @@ def fit_ladder(Z, x0, n_iter=30, rms_tol=1e-4, early_exit=True):
     return X.cpu().numpy(), rms.cpu().numpy()
 
 
-def benchmark(n_train=2_000_000, n_test=262_144, epochs=10, n_iter=30, rms_tol=1e-4, log=None):
+def benchmark(n_train=2_000_000, n_test=262_144, epochs=30, n_iter=30, rms_tol=1e-4, log=None):
     """Fitted voxels per second on noise-free held-out ladders, a voxel counting as fitted when its rms is at
     most rms_tol:
       nn          network start, LM with early exit, plain-x0 re-fit of the voxels still above rms_tol;
