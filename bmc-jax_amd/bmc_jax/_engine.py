@@ -24,11 +24,12 @@ def _events_key(events, reset_after_adc, compress):
         h.update(f.encode())
         h.update(str(a.shape).encode())
         h.update(a.astype(np.float64 if f != "type" else np.int32).tobytes())
-    return (h.hexdigest(), bool(reset_after_adc), bool(compress))
+    return (h.hexdigest(), bool(reset_after_adc), _native._level(compress))
 
 
 def plan_for(events, reset_after_adc=True, compress=True) -> _native.Plan:
-    """A device plan for an event table, cached on its exact contents."""
+    """A device plan for an event table, cached on its exact contents. compress: True / 2 (runs and
+    phase-shifted pulses reused), 1 (runs only), False / 0 (every event its own step)."""
     key = _events_key(events, reset_after_adc, compress)
     plan = _PLANS.get(key)
     if plan is not None:

@@ -103,7 +103,8 @@ def degree_and_squarings(l1):
 def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1):
     """Useful float64 flops of one bmc_simulate call, per voxel of prm [V, S] (summed over offsets and passes).
 
-    events: dict/SimEvent of (O, T) arrays; ops/row_ptr: the plan's program; slots: tangent slots.
+    events: dict/SimEvent of (O, T) arrays; ops/row_ptr: the plan's program (kinds 0 SEQ, 1 BLOCK,
+    2 KEEP counted as a BLOCK, 3 REUSE: the kept propagator applied once, rotated); slots: tangent slots.
     """
     ev = {k: np.asarray(getattr(events, k) if not isinstance(events, dict) else events[k])
           for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
@@ -118,6 +119,11 @@ def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1):
     for o in range(O):
         for oi in range(row_ptr[o], row_ptr[o + 1]):
             kind, e0, ln, cnt = (int(v) for v in ops[oi])
+            if kind == 3:                 # REUSE: the kept propagator applied once, rotated (2 x 6 flops per pool)
+                for ps in passes:
+                    T = len(ps)
+                    total += NA * (N + 3 * T * N) + N * (1 + T) * (NA - 1) + 2 * 6 * (1 + nc) * (1 + T)
+                continue
             sel = np.arange(e0, e0 + ln)
             dt = ev["dt"][o, sel].astype(np.float64)
             live = dt > 1e-9

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libbmc_hip.so")
 
 BMC_F32, BMC_F64 = 0, 1
-LM_EARLY_EXIT, LM_GRAPH = 1, 2
+LM_EARLY_EXIT, LM_GRAPH, LM_START_CHECK = 1, 2, 4
 MAX_CEST = 4
 MAX_TANGENTS = 16
 
@@ -67,6 +67,8 @@ def lib():
         L.bmc_plan_destroy.argtypes = [vp]
         L.bmc_plan_stats.argtypes = [vp, i64p, i64p, i64p]
         L.bmc_plan_program.argtypes = [vp, ctypes.c_int64, ip32, ip32]
+        L.bmc_plan_compile.argtypes = [ctypes.c_int, ctypes.c_int, ip32, dp, dp, dp, dp, ctypes.c_int, ctypes.c_int64,
+                                       ip32, dp, ip32, i64p]
         L.bmc_simulate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32, dp,
                                    ctypes.c_int, vp, vp, vp]
         L.bmc_lm_fit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ip32, dp, dp, dp,
@@ -78,6 +80,7 @@ def lib():
         L.bmc_simulate_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32,
                                            dp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int64, vp]
         for name in ("bmc_plan_create", "bmc_plan_create_ex", "bmc_plan_destroy", "bmc_plan_stats", "bmc_plan_program",
+                     "bmc_plan_compile",
                      "bmc_simulate", "bmc_simulate_trajectory", "bmc_lm_fit", "bmc_adjoint_workspace_size",
                      "bmc_simulate_adjoint"):
             getattr(L, name).restype = ctypes.c_int
@@ -110,10 +113,45 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _level(compress):
+    """Plan compression level from a bool (True = 2) or an int 0..2."""
+    if compress is True:
+        return 2
+    if compress is False or compress is None:
+        return 0
+    lv = int(compress)
+    if lv not in (0, 1, 2):
+        raise ValueError(f"compression level must be 0, 1 or 2, got {compress}")
+    return lv
+
+
+def compile_program(type_, dt, rf_amp, rf_phase, rf_freq, compress=2):
+    """The row programs of an event table, compiled on the host only (bmc_plan_compile; needs no GPU):
+    (ops [n_ops, 4] int32 = (kind, first event, length, count), rot [n_ops, 2] = (cos d, sin d),
+    row_ptr [O + 1] int32). Kinds: 0 SEQ, 1 BLOCK, 2 KEEP, 3 REUSE."""
+    L = lib()
+    t = np.ascontiguousarray(np.asarray(type_), dtype=np.int32)
+    if t.ndim != 2:
+        raise ValueError(f"events must be (n_offsets, n_events), got shape {t.shape}")
+    f = [np.ascontiguousarray(np.asarray(a, np.float64)) for a in (dt, rf_amp, rf_phase, rf_freq)]
+    O, T = t.shape
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    n = ctypes.c_int64()
+    args = [O, T, ip(t)] + [_dp(a) for a in f] + [_level(compress)]
+    check(L.bmc_plan_compile(*args, 0, None, None, None, ctypes.byref(n)), "bmc_plan_compile")
+    ops = np.zeros((max(n.value, 1), 4), np.int32)
+    rot = np.zeros((max(n.value, 1), 2), np.float64)
+    rp = np.zeros(O + 1, np.int32)
+    check(L.bmc_plan_compile(*args, max(n.value, 1), ip(ops), _dp(rot), ip(rp), ctypes.byref(n)), "bmc_plan_compile")
+    return ops[:n.value], rot[:n.value], rp
+
+
 class Plan:
     """A sequence compiled for the device (bmc_plan_create): events [O, T] and the row programs."""
 
-    def __init__(self, type_, dt, rf_amp, rf_phase, rf_freq, reset_after_adc=True, compress=True):
+    def __init__(self, type_, dt, rf_amp, rf_phase, rf_freq, reset_after_adc=True, compress=2):
+        """compress: 0 every event its own step; 1 runs of identical events chained; 2 (True) also pulses
+        that recur with a shifted RF phase kept and reused (bmc_plan_create_ex)."""
         L = lib()
         torch_cuda()
         t = np.ascontiguousarray(np.asarray(type_), dtype=np.int32)
@@ -128,8 +166,9 @@ class Plan:
         h = ctypes.c_void_p()
         check(L.bmc_plan_create_ex(self.n_offsets, self.n_events, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                    *[_dp(a) for a in self._arrays[1:]], 1 if reset_after_adc else 0,
-                                   1 if compress else 0, ctypes.byref(h)), "bmc_plan_create_ex")
+                                   _level(compress), ctypes.byref(h)), "bmc_plan_create_ex")
         self.compressed = bool(compress)
+        self.level = _level(compress)
         self._h = h
         self._arrays = None
 

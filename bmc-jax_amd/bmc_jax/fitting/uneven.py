@@ -107,7 +107,8 @@ class UnevenBatch:
         self.x_off = np.concatenate([[0], np.cumsum(nps)])
         self.scales = np.array(scales)
         self.plan_events = concat_events(rows)
-        self.plan = _engine.plan_for(self.plan_events)
+        # level 1: the persistent sweep holds no LDS slot for pulses kept across a row
+        self.plan = _engine.plan_for(self.plan_events, compress=1)
         self._keep = []
         prm_ptrs = (ctypes.c_void_p * 4)()
         m0_ptrs = (ctypes.c_void_p * 4)()

@@ -1,6 +1,7 @@
 """Builds libbmc_hip.so (the HIP kernels and the C ABI) in-tree, for gfx950.
 
     python bmc-jax_amd/build.py            # compile every .hip source and link the shared library
+    python bmc-jax_amd/build.py --incremental   # recompile only sources older than their inputs
 
 The library lands in bmc-jax_amd/bmc_jax/_lib/, where the Python layer loads it.
 """
@@ -33,7 +34,18 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, jobs: int = 4, defines=()) -> str:
+def _stale(obj, src):
+    """True when obj is missing or older than its source or any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    deps.append(os.path.join(ROOT, "include", "bmc_hip.h"))
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose: bool = False, jobs: int = 4, defines=(), incremental: bool = False) -> str:
+    """Compile every source (or, with incremental, only those older than their inputs) and link."""
     os.makedirs(OUT_DIR, exist_ok=True)
     objs_dir = os.path.join(HERE, "_build")
     os.makedirs(objs_dir, exist_ok=True)
@@ -42,6 +54,8 @@ def build(verbose: bool = False, jobs: int = 4, defines=()) -> str:
     for src in SOURCES:
         obj = os.path.join(objs_dir, src.replace(".hip", ".o"))
         objs.append(obj)
+        if incremental and not defines and not _stale(obj, os.path.join(CSRC, src)):
+            continue
         cmd = [hipcc()] + HIPCC_FLAGS + [f"-D{d}" for d in defines] + inc + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -67,4 +81,4 @@ def build(verbose: bool = False, jobs: int = 4, defines=()) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

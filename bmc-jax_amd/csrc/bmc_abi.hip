@@ -40,18 +40,112 @@ struct EvKeyHash {
   }
 };
 
+// Pulses of one row that recur with their RF phase shifted: maximal runs of RF events, grouped when
+// they agree event by event in type, dt, amplitude and frequency (bit-exact) and their phases differ
+// by one constant d on every sample with a nonzero amplitude (to kPhaseTol rad). The propagator of a
+// member is then R(d) P R(d)^T with P that of the group's first member (R: rotation about z of every
+// pool's (x, y)), which the kernel applies without rebuilding it.
+constexpr double kPhaseTol = 1e-12;
+
+struct PulseInfo {
+  int a = 0, b = 0;       // events [a, b)
+  int ref = 0;            // first sample with a nonzero amplitude (a if none)
+  int group = -1;
+  bool has_later = false; // a later member of the group starts after this pulse
+};
+
+std::vector<PulseInfo> find_pulses(const std::vector<EvKey>& row) {
+  const int T = (int)row.size();
+  std::vector<PulseInfo> pulses;
+  for (int e = 0; e < T;) {
+    if (row[e].type != BMC_EV_RF) {
+      e++;
+      continue;
+    }
+    PulseInfo p;
+    p.a = e;
+    while (e < T && row[e].type == BMC_EV_RF) e++;
+    p.b = e;
+    p.ref = p.a;
+    for (int t = p.a; t < p.b; t++)
+      if (row[t].amp != 0.0) {
+        p.ref = t;
+        break;
+      }
+    pulses.push_back(p);
+  }
+  // bucket by a hash of the phase-free contents, then compare phases within a bucket
+  auto same_shape = [&](const PulseInfo& x, const PulseInfo& y) {
+    if (x.b - x.a != y.b - y.a || x.ref - x.a != y.ref - y.a) return false;
+    for (int t = 0; t < x.b - x.a; t++) {
+      const EvKey &u = row[x.a + t], &v = row[y.a + t];
+      if (u.type != v.type || memcmp(&u.dt, &v.dt, 8) || memcmp(&u.amp, &v.amp, 8) || memcmp(&u.fr, &v.fr, 8))
+        return false;
+    }
+    const double d = row[x.ref].ph - row[y.ref].ph;
+    for (int t = 0; t < x.b - x.a; t++) {
+      const EvKey &u = row[x.a + t], &v = row[y.a + t];
+      if (u.amp != 0.0 && !(fabs((u.ph - v.ph) - d) <= kPhaseTol)) return false;
+    }
+    return true;
+  };
+  std::unordered_map<uint64_t, std::vector<int>> buckets;  // hash -> group leaders (pulse indices)
+  int n_groups = 0;
+  std::vector<int> leader_group;
+  for (int i = 0; i < (int)pulses.size(); i++) {
+    uint64_t h = 1469598103934665603ull ^ (uint64_t)(pulses[i].b - pulses[i].a);
+    for (int t = pulses[i].a; t < pulses[i].b; t++) {
+      uint64_t w[3];
+      memcpy(&w[0], &row[t].dt, 8);
+      memcpy(&w[1], &row[t].amp, 8);
+      memcpy(&w[2], &row[t].fr, 8);
+      for (int k = 0; k < 3; k++) h = (h ^ w[k]) * 1099511628211ull;
+    }
+    std::vector<int>& lead = buckets[h];
+    for (int l : lead)
+      if (same_shape(pulses[i], pulses[l])) {
+        pulses[i].group = pulses[l].group;
+        break;
+      }
+    if (pulses[i].group < 0) {
+      pulses[i].group = n_groups++;
+      lead.push_back(i);
+    }
+  }
+  std::vector<int> last_start(n_groups, -1);
+  for (const PulseInfo& p : pulses) last_start[p.group] = std::max(last_start[p.group], p.a);
+  for (PulseInfo& p : pulses) p.has_later = last_start[p.group] > p.a;
+  return pulses;
+}
+
 // Greedy run detection for one row: a block [i, i + p) repeated c >= 2 times back to back, with no
-// ADC inside, becomes one BLOCK op; everything else is SEQ. Candidate periods come from the next few
-// occurrences of event i's exact values.
-void compile_row(const std::vector<EvKey>& row, std::vector<Op>& ops, int64_t& built, int64_t& applied) {
+// ADC inside, becomes one BLOCK op. With level >= 2, a pulse that is not part of such a run and
+// recurs later with a shifted phase becomes KEEP (chained, kept), and its later members REUSE ops
+// with their rotation (cos d, sin d) appended to `rot`. Everything else is SEQ. Candidate periods come
+// from the next few occurrences of event i's exact values.
+void compile_row(const std::vector<EvKey>& row, int level, std::vector<Op>& ops, std::vector<double>& rot,
+                 int64_t& built, int64_t& applied) {
   const int T = (int)row.size();
   std::unordered_map<EvKey, std::vector<int>, EvKeyHash> pos;
   for (int e = 0; e < T; e++) pos[row[e]].push_back(e);
   std::vector<int> next_adc(T + 1, T);
   for (int e = T - 1; e >= 0; e--) next_adc[e] = (row[e].type == BMC_EV_ADC) ? e : next_adc[e + 1];
+  std::vector<PulseInfo> pulses;
+  std::vector<int> pulse_at(T, -1);  // pulse index starting at event e
+  if (level >= 2) {
+    pulses = find_pulses(row);
+    for (int i = 0; i < (int)pulses.size(); i++) pulse_at[pulses[i].a] = i;
+  }
+  int kept_group = -1;
+  double kept_phase = 0.0;
+  auto push = [&](Op op, double c, double sn) {
+    ops.push_back(op);
+    rot.push_back(c);
+    rot.push_back(sn);
+  };
   auto flush_seq = [&](int s0, int s1) {
     if (s1 > s0) {
-      ops.push_back(Op{OP_SEQ, s0, s1 - s0, 1});
+      push(Op{OP_SEQ, s0, s1 - s0, 1}, 1.0, 0.0);
       built += s1 - s0;
       applied += s1 - s0;
     }
@@ -78,14 +172,36 @@ void compile_row(const std::vector<EvKey>& row, std::vector<Op>& ops, int64_t& b
     }
     if (best_c >= 2) {
       flush_seq(seq_start, i);
-      ops.push_back(Op{OP_BLOCK, i, best_p, best_c});
+      push(Op{OP_BLOCK, i, best_p, best_c}, 1.0, 0.0);
       built += best_p;
       applied += best_c;
       i += best_p * best_c;
       seq_start = i;
-    } else {
-      i++;
+      continue;
     }
+    const int pi = pulse_at[i];
+    if (pi >= 0) {
+      const PulseInfo& pu = pulses[pi];
+      if (pu.group == kept_group) {
+        flush_seq(seq_start, i);
+        const double d = row[pu.ref].ph - kept_phase;
+        push(Op{OP_REUSE, pu.a, pu.b - pu.a, 1}, cos(d), sin(d));
+        applied += 1;
+        i = seq_start = pu.b;
+        continue;
+      }
+      if (pu.has_later) {
+        flush_seq(seq_start, i);
+        push(Op{OP_KEEP, pu.a, pu.b - pu.a, 1}, 1.0, 0.0);
+        kept_group = pu.group;
+        kept_phase = row[pu.ref].ph;
+        built += pu.b - pu.a;
+        applied += 1;
+        i = seq_start = pu.b;
+        continue;
+      }
+    }
+    i++;
   }
   flush_seq(seq_start, T);
 }
@@ -109,24 +225,29 @@ int bmc_device_count(void) {
 
 int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
                     const double* rf_phase, const double* rf_freq, int reset_after_adc, bmc_plan** out) {
-  return bmc_plan_create_ex(n_offsets, n_events, type, dt, rf_amp, rf_phase, rf_freq, reset_after_adc, 1, out);
+  return bmc_plan_create_ex(n_offsets, n_events, type, dt, rf_amp, rf_phase, rf_freq, reset_after_adc, 2, out);
 }
 
-int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
-                       const double* rf_phase, const double* rf_freq, int reset_after_adc, int compress,
-                       bmc_plan** out) {
-  if (!out || n_offsets <= 0 || n_events < 0 || (n_events > 0 && (!type || !dt || !rf_amp || !rf_phase || !rf_freq))) {
+}  // extern "C"
+
+namespace bmc {
+namespace {
+
+// The row programs of an event table (host only). Returns BMC_OK or BMC_EINVAL.
+int compile_plan(int O, int T, const int32_t* type, const double* dt, const double* rf_amp, const double* rf_phase,
+                 const double* rf_freq, int level, std::vector<Ev>* ev, std::vector<Op>& ops, std::vector<double>& rot,
+                 std::vector<int32_t>& row_ptr, int64_t& built, int64_t& applied) {
+  if (O <= 0 || T < 0 || (T > 0 && (!type || !dt || !rf_amp || !rf_phase || !rf_freq)) || level < 0 || level > 2) {
     set_error("bmc_plan_create: bad arguments");
     return BMC_EINVAL;
   }
-  *out = nullptr;
-  const int O = n_offsets, T = n_events;
-  std::vector<Ev> ev((size_t)O * T);
-  std::vector<Op> ops;
-  std::vector<int32_t> row_ptr(O + 1, 0);
-  int64_t built = 0, applied = 0;
+  if (ev) ev->assign((size_t)O * T, Ev{});
+  ops.clear();
+  rot.clear();
+  row_ptr.assign(O + 1, 0);
+  built = applied = 0;
+  std::vector<EvKey> row(T);
   for (int o = 0; o < O; o++) {
-    std::vector<EvKey> row(T);
     for (int e = 0; e < T; e++) {
       const size_t k = (size_t)o * T + e;
       if (type[k] < 0 || type[k] > 4 || !(dt[k] >= 0.0) || !isfinite(dt[k])) {
@@ -134,24 +255,79 @@ int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const d
         return BMC_EINVAL;
       }
       row[e] = EvKey{type[k], dt[k], rf_amp[k], rf_phase[k], rf_freq[k]};
-      ev[k] = Ev{dt[k], rf_amp[k], cos(rf_phase[k]), sin(rf_phase[k]), rf_freq[k], type[k], 0};
+      if (ev) (*ev)[k] = Ev{dt[k], rf_amp[k], cos(rf_phase[k]), sin(rf_phase[k]), rf_freq[k], type[k], 0};
     }
     row_ptr[o] = (int32_t)ops.size();
-    if (compress) {
-      compile_row(row, ops, built, applied);
+    if (level > 0) {
+      compile_row(row, level, ops, rot, built, applied);
     } else if (T > 0) {
       ops.push_back(Op{OP_SEQ, 0, T, 1});
+      rot.push_back(1.0);
+      rot.push_back(0.0);
       built += T;
       applied += T;
     }
   }
   row_ptr[O] = (int32_t)ops.size();
+  return BMC_OK;
+}
+
+}  // namespace
+}  // namespace bmc
+
+extern "C" {
+
+int bmc_plan_compile(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
+                     const double* rf_phase, const double* rf_freq, int compress, int64_t capacity, int32_t* ops_out,
+                     double* rot_out, int32_t* row_ptr_out, int64_t* n_ops_out) {
+  std::vector<Op> ops;
+  std::vector<double> rot;
+  std::vector<int32_t> row_ptr;
+  int64_t built, applied;
+  int rc = compile_plan(n_offsets, n_events, type, dt, rf_amp, rf_phase, rf_freq, compress, nullptr, ops, rot, row_ptr,
+                        built, applied);
+  if (rc != BMC_OK) return rc;
+  if (n_ops_out) *n_ops_out = (int64_t)ops.size();
+  if (!ops_out && !rot_out && !row_ptr_out) return BMC_OK;  // size query
+  if (!ops_out || !rot_out || !row_ptr_out || capacity < (int64_t)ops.size()) {
+    set_error("bmc_plan_compile: null output or capacity below the op count");
+    return BMC_EINVAL;
+  }
+  for (size_t i = 0; i < ops.size(); i++) {
+    ops_out[4 * i + 0] = ops[i].kind;
+    ops_out[4 * i + 1] = ops[i].e0;
+    ops_out[4 * i + 2] = ops[i].len;
+    ops_out[4 * i + 3] = ops[i].count;
+    rot_out[2 * i] = rot[2 * i];
+    rot_out[2 * i + 1] = rot[2 * i + 1];
+  }
+  for (size_t i = 0; i < row_ptr.size(); i++) row_ptr_out[i] = row_ptr[i];
+  return BMC_OK;
+}
+
+int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
+                       const double* rf_phase, const double* rf_freq, int reset_after_adc, int compress,
+                       bmc_plan** out) {
+  if (!out) {
+    set_error("bmc_plan_create: bad arguments");
+    return BMC_EINVAL;
+  }
+  *out = nullptr;
+  const int O = n_offsets, T = n_events;
+  std::vector<Ev> ev;
+  std::vector<Op> ops;
+  std::vector<double> rot;
+  std::vector<int32_t> row_ptr;
+  int64_t built = 0, applied = 0;
+  int rc = compile_plan(O, T, type, dt, rf_amp, rf_phase, rf_freq, compress, &ev, ops, rot, row_ptr, built, applied);
+  if (rc != BMC_OK) return rc;
   bmc_plan* pl = new bmc_plan();
   Plan& p = pl->p;
   p.O = O;
   p.T = T;
   p.reset_after_adc = reset_after_adc ? 1 : 0;
   p.compressed = compress ? 1 : 0;
+  p.has_keep = std::any_of(ops.begin(), ops.end(), [](const Op& op) { return op.kind == OP_KEEP; }) ? 1 : 0;
   p.n_ops = (int64_t)ops.size();
   p.n_built = built;
   p.n_applied = applied;
@@ -165,10 +341,14 @@ int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const d
   hipError_t e;
   if ((e = hipMalloc(&p.d_ev, sizeof(Ev) * std::max<size_t>(ev.size(), 1))) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&p.d_ops, sizeof(Op) * std::max<size_t>(ops.size(), 1))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&p.d_rot, sizeof(double) * std::max<size_t>(rot.size(), 2))) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&p.d_row_ptr, sizeof(int32_t) * row_ptr.size())) != hipSuccess) return fail(e);
   if (!ev.empty() && (e = hipMemcpy(p.d_ev, ev.data(), sizeof(Ev) * ev.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e);
   if (!ops.empty() && (e = hipMemcpy(p.d_ops, ops.data(), sizeof(Op) * ops.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e);
+  if (!rot.empty() && (e = hipMemcpy(p.d_rot, rot.data(), sizeof(double) * rot.size(), hipMemcpyHostToDevice)) !=
+                          hipSuccess)
     return fail(e);
   if ((e = hipMemcpy(p.d_row_ptr, row_ptr.data(), sizeof(int32_t) * row_ptr.size(), hipMemcpyHostToDevice)) !=
       hipSuccess)
@@ -183,6 +363,7 @@ int bmc_plan_destroy(bmc_plan* plan) {
   if (p.d_ev) (void)hipFree(p.d_ev);
   if (p.d_ops) (void)hipFree(p.d_ops);
   if (p.d_row_ptr) (void)hipFree(p.d_row_ptr);
+  if (p.d_rot) (void)hipFree(p.d_rot);
   delete plan;
   return BMC_OK;
 }
@@ -247,6 +428,8 @@ int fill_sim_args(const bmc_plan* plan, int V, int nc, int dtype, const void* pa
   a.ev = p.d_ev;
   a.ops = p.d_ops;
   a.row_ptr = p.d_row_ptr;
+  a.rot = p.d_rot;
+  a.keep = p.has_keep;
   a.O = p.O;
   a.T = p.T;
   a.reset_after_adc = p.reset_after_adc;
@@ -277,6 +460,9 @@ int fill_sim_args(const bmc_plan* plan, int V, int nc, int dtype, const void* pa
 // Tangents per pass for a tangent count (register budget of the kernel).
 #ifndef BMC_TANGENTS_PER_PASS
 #define BMC_TANGENTS_PER_PASS 2
+#endif
+#if BMC_TANGENTS_PER_PASS < 1 || BMC_TANGENTS_PER_PASS > 2
+#error "the simulate kernels are instantiated for 1 or 2 tangents per pass"
 #endif
 int tangents_per_pass(int P) { return P <= 0 ? 0 : (P < BMC_TANGENTS_PER_PASS ? P : BMC_TANGENTS_PER_PASS); }
 

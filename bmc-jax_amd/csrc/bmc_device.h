@@ -50,10 +50,10 @@ __device__ __forceinline__ double bcast(double x) {
   if constexpr (TEAM == 16) {
     return __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);  // row_newbcast:K
   } else if constexpr (TEAM == 8) {
-    // two teams share a DPP row: the lower half takes lane K, the upper half lane 8 + K
-    const double lo = __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);
-    const double hi = __builtin_amdgcn_mov_dpp(x, 0x150 + 8 + K, 0xf, 0xf, true);
-    return (threadIdx.x & 8) ? hi : lo;
+    // two teams share a DPP row: lanes 0-7 take lane K, lanes 8-15 lane 8 + K. The first move fills
+    // the row with lane K, the second overwrites lanes 8-15 only (bank_mask 0xc), so no select is needed.
+    const double all = __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);
+    return __builtin_amdgcn_update_dpp(all, x, 0x150 + 8 + K, 0xf, 0xc, false);
   } else {
     return __shfl(x, K, TEAM);
   }
@@ -594,17 +594,73 @@ __device__ __forceinline__ void event_cols(const Gen<NC>& G, const TanCo* tc, co
 }
 
 
+// M <- P M with P the kept propagator in LDS (lane j's column j at stride N + 1, tangent columns
+// after it), with tangents. Entries are read as they are used, so the kept block adds no registers
+// beyond the state's.
+template <int NC, int PT, int TEAM>
+__device__ __forceinline__ void apply_kept(const double* keep, double* M, double (*dM)[3 * (1 + NC) + 1], int j) {
+  constexpr int N = 3 * (1 + NC) + 1, NA = N + 1;
+  constexpr int PTA = PT > 0 ? PT : 1;
+  const bool own = j < NA;
+  double mj = (j == N) ? 1.0 : 0.0;
+  double dmj[PTA];
+#pragma unroll
+  for (int p = 0; p < PTA; p++) dmj[p] = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    if (j == i) {
+      mj = M[i];
+#pragma unroll
+      for (int p = 0; p < PTA; p++) dmj[p] = dM[p][i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    const double a = own ? keep[i * NA + j] : 0.0;
+#pragma unroll
+    for (int p = 0; p < PTA; p++) {
+      const double da = own ? keep[((1 + p) * N + i) * NA + j] : 0.0;
+      dM[p][i] = team_sum<TEAM>(fma(da, mj, a * dmj[p]));
+    }
+    M[i] = team_sum<TEAM>(a * mj);
+  }
+}
+
+// M <- R M for the rotation R = [[c, -s], [s, c]] of every pool's (x, y) about z, with tangents. The
+// generator of an RF event at phase phi + d is R(d) X(phi) R(d)^T: R commutes with the offset
+// precession, the transverse relaxation and the exchange terms, and turns the RF vector (w1x, w1y).
+template <int NC, int PTA>
+__device__ __forceinline__ void rotate_xy(double* M, double (*dM)[3 * (1 + NC) + 1], double c, double s) {
+  constexpr int NT = 1 + NC;
+#pragma unroll
+  for (int i = 0; i < NT; i++) {
+    const double x = M[i], y = M[NT + i];
+    M[i] = fma(c, x, -s * y);
+    M[NT + i] = fma(s, x, c * y);
+#pragma unroll
+    for (int p = 0; p < PTA; p++) {
+      const double dx = dM[p][i], dy = dM[p][NT + i];
+      dM[p][i] = fma(c, dx, -s * dy);
+      dM[p][NT + i] = fma(s, dx, c * dy);
+    }
+  }
+}
+
 // Runs one offset row's program (the event scan, simulate.py:80-132) on the team's state M and its
 // tangents dM: SEQ ops event by event, BLOCK ops chained into one propagator applied `count` times.
 // prm / st0: the team's parameters and start state (+ seeds) in LDS; st_park: LDS room for the state
 // while a block is chained. traj (optional): last_M after every event at traj_base + event * N.
-template <int NC, int PT, int TEAM>
+// With KEEP (a plan holding KEEP ops), KEEP ops also store the chained propagator in `keep` (the team's
+// LDS slot, (1 + PT) N (N + 1) doubles, lane j's column at stride N + 1) and REUSE ops apply it
+// conjugated by the op's rotation (rot[2 op], rot[2 op + 1]) = (cos d, sin d). Without it a KEEP op is
+// a BLOCK and the kept-pulse code is not compiled in: it would cost the other plans registers.
+template <int NC, int PT, int TEAM, bool KEEP = false>
 __device__ __forceinline__ void run_row(const double* prm, const double* st0, double* st_park, const int* kind,
                                         const int* kidx, const double* dwp, const Ev* ev_row, const Op* ops, int op0,
                                         int op1, int reset_after_adc, double nan_l1, int lane,
                                         double* M, double (*dM)[3 * (1 + NC) + 1], void* traj, long traj_base,
-                                        int traj_dtype) {
-  constexpr int NT = 1 + NC, N = 3 * NT + 1;
+                                        int traj_dtype, const double* rot = nullptr, double* keep = nullptr) {
+  constexpr int NT = 1 + NC, N = 3 * NT + 1, NA = N + 1;
   constexpr int PTA = PT > 0 ? PT : 1;
   auto load_start = [&]() {
 #pragma unroll
@@ -671,6 +727,16 @@ __device__ __forceinline__ void run_row(const double* prm, const double* st0, do
         }
         if (e.type == BMC_EV_ADC && reset_after_adc) pending = true;  // simulate.py:105-114
       }
+    } else if (KEEP && op.kind == OP_REUSE) {
+      // a block equal to the kept one up to an RF phase offset d: M <- R(d) P R(-d) M
+      if (pending) {
+        load_start();
+        pending = false;
+      }
+      const double c = rot[2 * oi], sn = rot[2 * oi + 1];
+      rotate_xy<NC, PTA>(M, dM, c, -sn);
+      apply_kept<NC, PT, TEAM>(keep, M, dM, lane);
+      rotate_xy<NC, PTA>(M, dM, c, sn);
     } else {
       if (pending) {
         load_start();
@@ -729,6 +795,14 @@ __device__ __forceinline__ void run_row(const double* prm, const double* st0, do
 #pragma unroll
             for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
           }
+        }
+      }
+      if (KEEP && op.kind == OP_KEEP && lane < NA) {  // each lane keeps its own column (read back by itself)
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          keep[i * NA + lane] = Pi[i];
+#pragma unroll
+          for (int p = 0; p < PTA; p++) keep[((1 + p) * N + i) * NA + lane] = dPi[p][i];
         }
       }
 #pragma unroll

@@ -21,11 +21,17 @@ struct Ev {
 //   kind 0 (SEQ):   events [e0, e0 + len) applied one by one to the state.
 //   kind 1 (BLOCK): the propagator of events [e0, e0 + len) is chained once, then applied `count`
 //                   times; the block holds no ADC.
+//   kind 2 (KEEP):  as BLOCK, and the chained propagator is also kept (the team's LDS slot) for later
+//                   REUSE ops of the same row.
+//   kind 3 (REUSE): events [e0, e0 + len) equal the kept block up to a constant RF phase offset d
+//                   (every sample's phase shifted by d): their propagator is R(d) P R(d)^T, with P the
+//                   kept one and R(d) the rotation by d about z of every pool's (x, y). Applied once
+//                   as M <- R(d) P R(-d) M; the op's (cos d, sin d) is in the plan's rotation table.
 struct Op {
   int32_t kind, e0, len, count;
 };
 
-enum { OP_SEQ = 0, OP_BLOCK = 1 };
+enum { OP_SEQ = 0, OP_BLOCK = 1, OP_KEEP = 2, OP_REUSE = 3 };
 
 // Tangent kinds: how the generator depends on a packed slot (matrix.py:83-295 differentiated).
 enum TanKind {
@@ -79,6 +85,8 @@ struct SimArgs {
   long team0;             // first team of this launch (a launch covers at most 2^31 threads)
   const int32_t* vlist;   // [<= V] or null: simulate only these voxels (compacted list)
   const int32_t* n_vlist; // device count of vlist
+  const double* rot;      // [n_ops, 2] (cos d, sin d) of REUSE ops
+  int keep;               // the plan holds KEEP ops: the launch gives every team an LDS slot for one propagator
 };
 
 struct Plan {
@@ -86,8 +94,10 @@ struct Plan {
   Ev* d_ev = nullptr;
   Op* d_ops = nullptr;
   int32_t* d_row_ptr = nullptr;
+  double* d_rot = nullptr;  // [n_ops, 2]
   int64_t n_ops = 0, n_built = 0, n_applied = 0;
   int compressed = 1;
+  int has_keep = 0;
   std::vector<Op> h_ops;          // host copy of the program
   std::vector<int32_t> h_row_ptr; // host copy of the row offsets
 };

@@ -8,7 +8,8 @@
 //      + 1e-9 I) (+ diag(1 - free)), delta = A^-1 (-g) by LU with partial pivoting, x_t = clip(x + delta)
 //   3. sim_kernel at x_t, primal only                    -> z_t [V, O]
 //   4. accept_kernel: c_t = |z_t - z_meas|^2; accept iff c_t < c (lam * 0.5) else reject (lam * 3)
-// The iteration counter lives on the device so a replayed graph advances it.
+// The iteration counter lives on the device so a replayed graph advances it. With BMC_LM_START_CHECK a
+// primal sweep at the start values first stops the problems whose start already meets the rms tolerance.
 #include <math.h>
 #include <string.h>
 
@@ -195,6 +196,27 @@ __global__ void __launch_bounds__(64) accept_kernel(AcArgs a) {
   }
 }
 
+// The start check (BMC_LM_START_CHECK): cost at the start values from a primal sweep; a problem whose
+// rms is already at most rms_tol is stopped before its first Jacobian (it keeps x0, 0 iterations).
+__global__ void __launch_bounds__(64) start_kernel(AcArgs a) {
+  const int v = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (v >= a.V || !a.s.active[v]) return;
+  double c = 0.0;
+  const double* zv = a.zt + (long)v * a.O;
+  for (int o = lane; o < a.O; o += 64) {
+    double r = zv[o] - ldr(a.zmeas, (long)v * a.O + o, a.dtype);
+    c = fma(r, r, c);
+  }
+  c = wave_sum(c);
+  if (lane != 0) return;
+  a.s.cost[v] = c;
+  if (sqrt(c / (double)a.O) <= a.rms_tol) {
+    a.s.active[v] = 0;
+    a.s.jac_live[v] = 0;
+  }
+}
+
 // list <- indices v with flag[v] != 0 (any order), count <- their number (count cleared beforehand).
 // One atomic per wave: the wave's hits are ranked by ballot.
 __global__ void __launch_bounds__(256) compact_kernel(const int32_t* flag, int V, int32_t* list, int32_t* count) {
@@ -357,6 +379,17 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
 
   hipLaunchKernelGGL(init_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, st, V, lam0);
   BMC_HIP_CHECK(hipGetLastError());
+  if ((flags & BMC_LM_START_CHECK) && (flags & BMC_LM_EARLY_EXIT)) {
+    SimArgs s0 = sp;  // primal at the start values of every problem
+    s0.xw = x_dev;
+    hipLaunchKernelGGL(clear2_kernel, dim3(1), dim3(64), 0, stream, cnt);
+    hipLaunchKernelGGL(compact_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, (const int32_t*)st.active, V,
+                       (int32_t*)bla.p, cnt + 1);
+    BMC_HIP_CHECK(hipGetLastError());
+    if ((rc = launch_sim(n_cest, s0, stream)) != BMC_OK) return rc;
+    hipLaunchKernelGGL(start_kernel, dim3(V), dim3(64), 0, stream, ac);
+    BMC_HIP_CHECK(hipGetLastError());
+  }
 
   auto iteration = [&]() -> int {
     int r;

@@ -17,7 +17,9 @@
 //    (s == 0) acts on the state directly with the Taylor series; otherwise its propagator is built
 //    by columns and applied with a team reduction.
 //  * Runs of identical events (a repeated pulse and its gap) are chained into one propagator and
-//    applied `count` times (the row program built on the host). An event that needs no squaring is
+//    applied `count` times (the row program built on the host). A pulse that recurs later in the row
+//    with every sample's RF phase shifted by the same d (the phase accumulated by a scanner's pulse
+//    train) is chained once, kept in LDS, and applied as R(d) P R(-d), R a rotation about z. An event that needs no squaring is
 //    chained by running its Taylor series on each lane's column of the chained propagator, which
 //    needs no dense product at all; only events with squarings form their own propagator first.
 // Tangents travel beside every quantity (dual numbers): d(X v) = dX v + X dv, where dX is the
@@ -35,15 +37,21 @@ namespace bmc {
 #define BMC_SIM_WAVES_PER_SIMD 2
 #endif
 
+// Doubles of one team's keep slot: the propagator and its PT tangents, N rows by N + 1 columns.
+template <int NC, int PT>
+__host__ __device__ constexpr long keep_slot_doubles() {
+  return (long)(1 + (PT > 0 ? PT : 1)) * (3 * (1 + NC) + 1) * (3 * (1 + NC) + 2);
+}
+
 // One team's (voxel, offset row, pass) of a launch with `total` teams over n_vox voxels (voxel
 // index through a.vlist when a compacted list is given). Every thread of the block calls it: it
 // holds block barriers before its first early exit.
-template <int NC, int PT, int TEAM, bool LISTED>
+template <int NC, int PT, int TEAM, bool LISTED, bool KEEP>
 __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total, int n_vox,
                                          double (*sh_prm)[PrmIdx<NC>::size],
                                          double (*sh_state)[(1 + (PT > 0 ? PT : 1)) * (3 * (1 + NC) + 1)],
                                          double (*sh_start)[(1 + (PT > 0 ? PT : 1)) * (3 * (1 + NC) + 1)],
-                                         int* sh_tk, int* sh_ti) {
+                                         int* sh_tk, int* sh_ti, double* keep) {
   constexpr int NT = 1 + NC, N = 3 * NT + 1;
   using I = PrmIdx<NC>;
   constexpr int PTA = PT > 0 ? PT : 1;
@@ -119,9 +127,10 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   // state (identical on every lane of the team); the start state is reloaded from memory on a reset
   double M[N];
   double dM[PTA][N];
-  run_row<NC, PT, TEAM>(prm, st0, sh_state[tib], kind, kidx, dwp, a.ev + (long)o * a.T, a.ops, a.row_ptr[o],
+  run_row<NC, PT, TEAM, KEEP>(prm, st0, sh_state[tib], kind, kidx, dwp, a.ev + (long)o * a.T, a.ops, a.row_ptr[o],
                         a.row_ptr[o + 1], a.reset_after_adc, a.nan_l1, lane, M, dM,
-                        (a.traj && !dead && q == 0) ? a.traj : nullptr, ((long)v * a.O + o) * a.T * N, a.dtype);
+                        (a.traj && !dead && q == 0) ? a.traj : nullptr, ((long)v * a.O + o) * a.T * N, a.dtype,
+                        a.rot, keep);
   if (dead || lane != 0) return;
   const long vo = (long)v * a.O + o;
   if (q == 0) {
@@ -144,7 +153,7 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   }
 }
 
-template <int NC, int PT, int TEAM>
+template <int NC, int PT, int TEAM, bool KEEP>
 __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArgs a) {
   constexpr int N = 3 * (1 + NC) + 1;
   constexpr int TPB = 256 / TEAM;
@@ -153,14 +162,16 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArg
   __shared__ double sh_state[TPB][(1 + PTA) * N];  // the state, parked while a block is chained
   __shared__ double sh_start[TPB][(1 + PTA) * N];  // the start state and its tangent seeds
   __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
+  extern __shared__ double dyn_keep[];  // TPB keep slots when the plan holds KEEP ops
   const int tib = threadIdx.x / TEAM;
-  sim_team<NC, PT, TEAM, false>(a, a.team0 + (long)blockIdx.x * TPB + tib, (long)a.V * a.O * a.n_pass, a.V, sh_prm,
-                         sh_state, sh_start, sh_tk, sh_ti);
+  double* keep = KEEP ? dyn_keep + (long)tib * keep_slot_doubles<NC, PT>() : nullptr;
+  sim_team<NC, PT, TEAM, false, KEEP>(a, a.team0 + (long)blockIdx.x * TPB + tib, (long)a.V * a.O * a.n_pass, a.V, sh_prm,
+                         sh_state, sh_start, sh_tk, sh_ti, keep);
 }
 
 // The same over a compacted voxel list (a.vlist, device count a.n_vlist): a fixed grid strides over the
 // live teams; the trip count is block-uniform.
-template <int NC, int PT, int TEAM>
+template <int NC, int PT, int TEAM, bool KEEP>
 __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_list_kernel(SimArgs a) {
   constexpr int N = 3 * (1 + NC) + 1;
   constexpr int TPB = 256 / TEAM;
@@ -169,25 +180,34 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_list_kernel(S
   __shared__ double sh_state[TPB][(1 + PTA) * N];
   __shared__ double sh_start[TPB][(1 + PTA) * N];
   __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
+  extern __shared__ double dyn_keep[];
   const int tib = threadIdx.x / TEAM;
+  double* keep = KEEP ? dyn_keep + (long)tib * keep_slot_doubles<NC, PT>() : nullptr;
   const int nv = *a.n_vlist;
   const long total = (long)nv * a.O * a.n_pass;
   for (long base = (long)blockIdx.x * TPB; base < total; base += (long)gridDim.x * TPB) {
-    sim_team<NC, PT, TEAM, true>(a, base + tib, total, nv, sh_prm, sh_state, sh_start, sh_tk, sh_ti);
+    sim_team<NC, PT, TEAM, true, KEEP>(a, base + tib, total, nv, sh_prm, sh_state, sh_start, sh_tk, sh_ti, keep);
     __syncthreads();  // the next item overwrites the block's LDS
   }
 }
 
-template <int NC, int PT>
-static int launch_nc_pt(const SimArgs& a, hipStream_t stream) {
+template <int NC, int PT, bool KEEP>
+static int launch_nc_pt_k(const SimArgs& a, hipStream_t stream) {
   constexpr int NA = 3 * (1 + NC) + 2;
   constexpr int TEAM = NA <= 8 ? 8 : (NA <= 16 ? 16 : 32);  // one lane per propagator column
   constexpr int TPB = 256 / TEAM;
   const long teams = (long)a.V * a.O * a.n_pass;
+  const size_t dyn = KEEP ? (size_t)TPB * keep_slot_doubles<NC, PT>() * sizeof(double) : 0;
+  if (dyn > 65536) {  // gfx950 gives a workgroup up to 160 KiB of LDS; dynamic LDS above 64 KiB is opt-in
+    BMC_HIP_CHECK(hipFuncSetAttribute((const void*)sim_kernel<NC, PT, TEAM, KEEP>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    BMC_HIP_CHECK(hipFuncSetAttribute((const void*)sim_list_kernel<NC, PT, TEAM, KEEP>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+  }
   if (a.vlist) {  // persistent grid over a device-side voxel list of at most V entries
     const long blocks = std::min<long>((teams + TPB - 1) / TPB, 2048);
     if (blocks <= 0) return BMC_OK;
-    hipLaunchKernelGGL((sim_list_kernel<NC, PT, TEAM>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((sim_list_kernel<NC, PT, TEAM, KEEP>), dim3((unsigned)blocks), dim3(256), dyn, stream, a);
     BMC_HIP_CHECK(hipGetLastError());
     return BMC_OK;
   }
@@ -197,10 +217,16 @@ static int launch_nc_pt(const SimArgs& a, hipStream_t stream) {
     const long blocks = std::min(kMaxBlocks, (teams - t0 + TPB - 1) / TPB);
     SimArgs c = a;
     c.team0 = t0;
-    hipLaunchKernelGGL((sim_kernel<NC, PT, TEAM>), dim3((unsigned)blocks), dim3(256), 0, stream, c);
+    hipLaunchKernelGGL((sim_kernel<NC, PT, TEAM, KEEP>), dim3((unsigned)blocks), dim3(256), dyn, stream, c);
     BMC_HIP_CHECK(hipGetLastError());
   }
   return BMC_OK;
+}
+
+// Plans with kept pulses run the kernels that hold the kept-pulse code and its LDS slots.
+template <int NC, int PT>
+static int launch_nc_pt(const SimArgs& a, hipStream_t stream) {
+  return a.keep ? launch_nc_pt_k<NC, PT, true>(a, stream) : launch_nc_pt_k<NC, PT, false>(a, stream);
 }
 
 template <int NC>
@@ -212,10 +238,8 @@ static int launch_nc(const SimArgs& a, hipStream_t stream) {
       return launch_nc_pt<NC, 1>(a, stream);
     case 2:
       return launch_nc_pt<NC, 2>(a, stream);
-    case 3:
-      return launch_nc_pt<NC, 3>(a, stream);
     default:
-      set_error("simulate: tangents per pass must be 0..3");
+      set_error("simulate: tangents per pass must be 0..2");
       return BMC_EINVAL;
   }
 }

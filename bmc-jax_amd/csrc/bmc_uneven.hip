@@ -483,6 +483,10 @@ extern "C" int bmc_ubatch_create(const bmc_plan* plan, int n_problems, int dtype
     set_error("bmc_ubatch_create: bad argument");
     return BMC_EINVAL;
   }
+  if (plan->p.has_keep) {  // the one-wave persistent workgroups hold no LDS slot for a kept pulse
+    set_error("bmc_ubatch_create: the plan keeps pulses for reuse (compression level 2); create it with level <= 1");
+    return BMC_ENOTSUP;
+  }
   auto b = std::make_unique<bmc_ubatch>();
   b->plan = plan;
   b->n_prob = n_problems;

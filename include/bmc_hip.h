@@ -77,16 +77,28 @@ int bmc_device_count(void);
 typedef struct bmc_plan bmc_plan;
 int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
                     const double* rf_phase, const double* rf_freq, int reset_after_adc, bmc_plan** out);
-/* As bmc_plan_create; compress = 0 keeps every event in order as its own step (no chained runs), which
- * bmc_simulate_trajectory needs. */
+/* As bmc_plan_create with a compression level: 0 keeps every event in order as its own step (no chained
+ * runs), which bmc_simulate_trajectory needs; 1 chains runs of identical events repeated back to back;
+ * 2 (bmc_plan_create's level) also keeps a pulse that recurs later in its row with every sample's RF
+ * phase shifted by one constant d (a scanner train's accumulated phase, sequences/pulseq.py:60-66)
+ * and applies it again rotated by d about z instead of rebuilding it. */
 int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
                        const double* rf_phase, const double* rf_freq, int reset_after_adc, int compress,
                        bmc_plan** out);
+/* The row programs bmc_plan_create_ex would build, on the host only (no device call): ops_out
+ * [n_ops, 4] and rot_out [n_ops, 2] as bmc_plan_program describes them, row_ptr_out [n_offsets + 1].
+ * With all three outputs NULL it only stores the op count in n_ops_out (a size query). Replaces
+ * nothing in the reference; it lets a caller inspect a compilation without a GPU. */
+int bmc_plan_compile(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
+                     const double* rf_phase, const double* rf_freq, int compress, int64_t capacity, int32_t* ops_out,
+                     double* rot_out, int32_t* row_ptr_out, int64_t* n_ops_out);
 int bmc_plan_destroy(bmc_plan* plan);
 /* Statistics of a plan: ops and the number of events whose propagator is built per offset row. */
 int bmc_plan_stats(const bmc_plan* plan, int64_t* n_ops, int64_t* n_built, int64_t* n_applied);
-/* The row programs: ops_out [n_ops, 4] = (kind 0 SEQ / 1 BLOCK, first event, length, repeat count),
- * row_ptr_out [n_offsets + 1]. capacity is the number of ops ops_out can hold. */
+/* The row programs: ops_out [n_ops, 4] = (kind, first event, length, repeat count), row_ptr_out
+ * [n_offsets + 1]. Kinds: 0 SEQ (events one by one), 1 BLOCK (chained once, applied `count` times),
+ * 2 KEEP (as BLOCK, and kept), 3 REUSE (the kept block rotated by the op's phase offset d: (cos d,
+ * sin d) in bmc_plan_compile's rot_out). capacity is the number of ops ops_out can hold. */
 int bmc_plan_program(const bmc_plan* plan, int64_t capacity, int32_t* ops_out, int32_t* row_ptr_out);
 
 /*
@@ -136,11 +148,14 @@ int bmc_simulate_trajectory(const bmc_plan* plan, int n_voxels, int n_cest, int 
  *   x_traj_dev   [V, n_iter, P] float64 or NULL
  *   flags        BMC_LM_EARLY_EXIT: stop a problem once it has converged (never the reference's
  *                behaviour, which runs every iteration); BMC_LM_GRAPH: replay the iteration as a
- *                captured hipGraph.
+ *                captured hipGraph; BMC_LM_START_CHECK (with BMC_LM_EARLY_EXIT): simulate every start
+ *                first and stop, before any Jacobian, the problems whose start rms is at most
+ *                early_rms (they return x0 after 0 iterations).
  *   n_iter_done_dev [V] int32 or NULL        iterations each problem ran
  */
 #define BMC_LM_EARLY_EXIT 1u
 #define BMC_LM_GRAPH 2u
+#define BMC_LM_START_CHECK 4u
 int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
                const void* m0_dev, int n_params, const int32_t* slot, const double* scale, const double* lo,
                const double* hi, int jac_component, const void* z_meas_dev, double* x_dev, int n_iter,

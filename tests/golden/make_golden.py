@@ -6,6 +6,12 @@ cfg1_f64.npz   configuration 1 (2-pool CW, 61 offsets): final water Mz and d/d(k
 cfg2_slice.npz configuration 2's full pulsed train (50 Gaussian pulses x 200 steps, 10,053 events) on a
                thin slice: 3 voxels x 6 offsets, with the 5 free values (k_amide, f_amide, water_r1,
                water_r2, b0_inhom). The packed parameters and offsets are stored alongside.
+scanner_rows.npz  two rows (-4.0 and +2.0 ppm) of a converted scanner sequence shipped with the
+               reference (examples/data/events/quesp_sim_b1_1uT_events.npz: 20 shaped pulses x 200 samples,
+               each pulse at its own accumulated RF phase, 4041 events), as float32 values the way the
+               reference's loader casts them, and two tube-like voxels (one CEST pool, start magnetisation
+               0.557): final states and d/d(k, f, pool R2). Written only where the reference is mounted;
+               the fixture itself is data (event values and oracle outputs).
 The oracle itself is pinned by tests/test_oracle.py (closed forms, scipy, independent routes).
 """
 import os
@@ -53,6 +59,24 @@ def main():
     m2, dm2 = OC.simulate(prm, 2, ev2, m0, slots=idx2)
     np.savez(os.path.join(HERE, "cfg2_slice.npz"), params=prm, ppm=ppm, m0=m0, z=m2[..., 6], jac=dm2[..., 6],
              slots=np.array(idx2))
+    ref = "/root/reference/examples/data/events/quesp_sim_b1_1uT_events.npz"
+    if os.path.exists(ref):
+        with np.load(ref) as d:
+            offsets = np.asarray(d["offsets_ppm"])
+            rows = [int(np.argmin(np.abs(offsets - v))) for v in (-4.0, 2.0)]
+            evs = {k: np.asarray(d[k])[rows].astype(np.float32).astype(np.float64)
+                   for k in ("dt", "rf_amp", "rf_phase", "rf_freq")}
+            evs["type"] = np.asarray(d["type"])[rows].astype(np.int32)
+        tubes = [O.make_params(1 / 0.35, 1 / 0.5605, pools=[dict(f=5e-3, k=900.0, dw=1.0, t1=1.0, t2=0.01)],
+                               b0=2.8936, rel_b1=0.97),
+                 O.make_params(1 / 0.4, 1 / 0.5605, pools=[dict(f=3.2e-3, k=350.0, dw=1.9, t1=1.0, t2=0.02)],
+                               b0=2.8936, b0_inhom=0.02, rel_b1=1.03)]
+        prm = np.stack([OC.pack(t) for t in tubes])
+        m0 = np.stack([O.initial_state(t, m_start=0.557) for t in tubes])
+        idx = [OC.slot_index(f, 0, 1) for f in ("cest_k", "cest_f", "cest_r2")]
+        m3, dm3 = OC.simulate(prm, 1, evs, m0, slots=idx)
+        np.savez_compressed(os.path.join(HERE, "scanner_rows.npz"), params=prm, m0=m0, slots=np.array(idx),
+                            offsets_ppm=offsets[rows], m=m3, jac=dm3[..., 4], **evs)
     print("wrote", os.listdir(HERE))
 
 

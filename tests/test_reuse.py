@@ -1,0 +1,147 @@
+"""Pulses reused across a row with a shifted RF phase (plan compression level 2), and the event-cache
+format. CPU only: the rotation identity on the oracle, the host compilation (bmc_plan_compile), the
+cache reader and writer.
+
+A converted scanner sequence gives every pulse of a train the phase accumulated by the pulses before it
+(the reference's sequences/pulseq.py:56-66, 118-130), so no two pulses are equal and a plan can only
+reuse one by rotating it: exp(X(phi + d) dt) = R(d) exp(X(phi) dt) R(d)^T, R(d) the rotation by d of
+every pool's (x, y). That identity is checked on the oracle's generator (bmc_matrix restates
+matrix.py:83-295) and exponential, independently of the kernels.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import bmc_oracle as O
+from helpers import four_pool_params
+
+REF_EVENTS = "/root/reference/examples/data/events"
+OP_SEQ, OP_BLOCK, OP_KEEP, OP_REUSE = 0, 1, 2, 3
+
+
+def rot_full(N, nt, d):
+    """Rotation by d of every pool's (x, y) on the augmented state (x block [0, nt), y block [nt, 2 nt))."""
+    R = np.eye(N + 1)
+    c, s = np.cos(d), np.sin(d)
+    for i in range(nt):
+        R[i, i], R[i, nt + i], R[nt + i, i], R[nt + i, nt + i] = c, -s, s, c
+    return R
+
+
+@pytest.mark.parametrize("d", [0.3, 2.0587284868, -5.1])
+def test_a_phase_offset_is_a_rotation_about_z(d):
+    p = four_pool_params(k_am=200.0, f_am=3e-3, b0i=0.1, rel_b1=0.9)
+    nt = 1 + len(p["cest_k"])
+    N = 3 * nt + 1
+    for amp, ph, freq, dt in ((101.3, 0.4, -300.0, 5e-4), (12.0, -1.1, 616.0, 0.02), (40.0, 0.0, 0.0, 2.0)):
+        A0, C0 = O.bmc_matrix(p, amp, ph, freq)
+        A1, C1 = O.bmc_matrix(p, amp, ph + d, freq)
+        E0 = O.expm64(O.augmented(A0, C0, dt))
+        E1 = O.expm64(O.augmented(A1, C1, dt))
+        R = rot_full(N, nt, d)
+        np.testing.assert_allclose(R @ O.augmented(A0, C0, dt) @ R.T, O.augmented(A1, C1, dt), rtol=0, atol=1e-12)
+        assert np.max(np.abs(R @ E0 @ R.T - E1)) < 1e-13 * max(1.0, np.abs(E1).max())
+        # a plausible mistake, the inverse rotation, is not the same thing
+        assert np.max(np.abs(R.T @ E0 @ R - E1)) > 1e-6
+
+
+def _compile(ev, level):
+    from bmc_jax import _native
+    return _native.compile_program(ev.type, ev.dt, ev.rf_amp, ev.rf_phase, ev.rf_freq, level)
+
+
+def _pulse_starts(typ_row):
+    rf = typ_row == 1
+    return np.flatnonzero(rf & ~np.r_[False, rf[:-1]])
+
+
+def test_scanner_train_keeps_one_pulse_per_row():
+    from bmc_jax.sequences import scanner_train
+    ev = scanner_train(np.linspace(-5, 5, 7), 1.3, n_pulses=6, n_samples=30, m0_ppm=-300.0)
+    ops, rot, rp = _compile(ev, 2)
+    for o in range(ev.type.shape[0]):
+        row = ops[rp[o]:rp[o + 1]]
+        kinds = row[:, 0]
+        assert np.sum(kinds == OP_KEEP) == 1 and np.sum(kinds == OP_REUSE) == 5
+        starts = _pulse_starts(ev.type[o])
+        assert list(row[kinds >= OP_KEEP, 1]) == list(starts)
+        assert np.all(row[kinds >= OP_KEEP, 2] == 30)
+        ph = ev.rf_phase[o, starts]
+        d = ph[1:] - ph[0]
+        np.testing.assert_allclose(rot[rp[o]:rp[o + 1]][kinds == OP_REUSE], np.stack([np.cos(d), np.sin(d)], 1),
+                                   atol=1e-15)
+        # the ops tile the row exactly, in order
+        assert row[0, 1] == 0 and np.all(row[1:, 1] == row[:-1, 1] + row[:-1, 2] * np.where(row[:-1, 0] == 1,
+                                                                                              row[:-1, 3], 1))
+        assert row[-1, 1] + row[-1, 2] == ev.type.shape[1]
+    ops1, _, _ = _compile(ev, 1)
+    assert np.all(ops1[:, 0] == OP_SEQ)   # no pulse repeats back to back: nothing to chain at level 1
+    ops0, _, rp0 = _compile(ev, 0)
+    assert len(ops0) == ev.type.shape[0] and np.all(ops0[:, 2] == ev.type.shape[1])
+
+
+def test_reuse_needs_one_phase_offset_for_the_whole_pulse():
+    from bmc_jax.sequences import scanner_train
+    ev = scanner_train(np.array([-2.0, 3.0]), 1.0, n_pulses=5, n_samples=20)
+    ph = np.array(ev.rf_phase, copy=True)
+    s = _pulse_starts(ev.type[0])
+    ph[0, s[2] + 7] += 1e-6                       # one sample of pulse 2 in row 0 off the common offset
+    ev = ev._replace(rf_phase=ph)
+    ops, rot, rp = _compile(ev, 2)
+    r0, r1 = ops[rp[0]:rp[1]], ops[rp[1]:rp[2]]
+    assert s[2] not in r0[r0[:, 0] >= OP_KEEP, 1]          # pulse 2 runs event by event
+    assert np.sum(r0[:, 0] == OP_REUSE) == 3 and np.sum(r1[:, 0] == OP_REUSE) == 4
+    amp = np.array(ev.rf_amp, copy=True)
+    amp[1, s[3] + 1] *= 1.0 + 1e-12                          # any amplitude change breaks the match too
+    ops, _, rp = _compile(ev._replace(rf_amp=amp), 2)
+    assert np.sum(ops[rp[1]:rp[2], 0] == OP_REUSE) == 3
+
+
+def test_identical_back_to_back_pulses_still_chain_as_blocks():
+    """The level-1 runs take precedence: a train of identical pulse + gap units is one BLOCK."""
+    from bmc_jax.sequences import pulsed_train
+    ev = pulsed_train(np.array([1.0, -2.0]), n_pulses=8, n_samples=12)
+    for level in (1, 2):
+        ops, _, rp = _compile(ev, level)
+        kinds = ops[:, 0]
+        assert np.sum(kinds == OP_BLOCK) == 2 and np.all(ops[kinds == OP_BLOCK, 3] == 8)
+        assert not np.any(kinds >= OP_KEEP)
+
+
+def test_event_cache_round_trip(tmp_path):
+    from bmc_jax.sequences import load_events, save_events, scanner_train
+    ppm = np.r_[-300.0, np.linspace(-5, 5, 11)]
+    ev = scanner_train(ppm[1:], 0.8, n_pulses=3, n_samples=10, m0_ppm=-300.0)
+    f = tmp_path / "rung_events.npz"
+    save_events(f, ev, ppm, max_pulse_samples=10, b1_uT=0.8)
+    back, meta = load_events(f, dtype=np.float64, return_meta=True)
+    assert back.type.shape == (11, ev.type.shape[1])
+    for k in ev._fields:
+        np.testing.assert_array_equal(np.asarray(getattr(back, k)), np.asarray(getattr(ev, k))[1:])
+    np.testing.assert_array_equal(meta["offsets_ppm"], ppm[1:])
+    assert int(meta["max_pulse_samples"]) == 10 and float(meta["b1_uT"]) == 0.8
+    keep = np.abs(ppm[1:]) >= 1.0
+    sub, meta = load_events(f, keep=keep, return_meta=True)
+    assert sub.type.shape[0] == keep.sum() and sub.dt.dtype == np.float32
+    np.testing.assert_array_equal(meta["offsets_ppm"], ppm[1:][keep])
+    np.savez(tmp_path / "bad.npz", dt=np.zeros((2, 3)))
+    with pytest.raises(ValueError, match="not an event cache"):
+        load_events(tmp_path / "bad.npz")
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EVENTS), reason="the reference's event caches are not mounted")
+def test_the_reference_caches_compile_to_kept_pulses():
+    """The shipped scanner conversions (examples/data/events/*.npz): every pulse after a row's first is
+    applied as a rotated copy of it."""
+    from bmc_jax.sequences import load_events
+    for name in sorted(os.listdir(REF_EVENTS)):
+        ev, meta = load_events(os.path.join(REF_EVENTS, name), return_meta=True)
+        assert int(meta["max_pulse_samples"]) == 200
+        assert ev.type.shape[0] == len(meta["offsets_ppm"]) == 101
+        ops, rot, rp = _compile(ev, 2)
+        for o in (0, 50, 100):
+            row = ops[rp[o]:rp[o + 1]]
+            n_pulses = len(_pulse_starts(np.asarray(ev.type[o])))
+            assert np.sum(row[:, 0] == OP_KEEP) == 1, name
+            assert np.sum(row[:, 0] == OP_REUSE) == n_pulses - 1, name
