@@ -402,6 +402,81 @@ def bench_scanner(args):
     print(json.dumps(line), flush=True)
 
 
+REF_JOINT_FIT_S = 7200.0    # examples/README.md:65 of the reference: about 2 h on one RTX 3070
+
+
+def bench_joint(args):
+    """The reference's tutorial 10 (examples/10_shared_water.py) at its shape, on synthetic data: seven
+    tubes (a water-only tube plus the six tubes of the scanner workload) on the same 656-row QUESP
+    ladder. Step 1 fits k, f and pool R2 of the six pool tubes with the water held (fit, 60 iterations);
+    step 2 is fit_joint over all seven, 12 LM steps, with the water R2 factor, M_start and the B0 spread
+    (Gauss-Hermite, 15 nodes) shared by every tube. Z is the float64 model at truths with those shared
+    values (R2 x 1.1, M_start 0.60, spread 3 Hz) plus N(0, 1e-3)."""
+    import torch
+    from dataclasses import replace
+    from bmc_jax import _engine
+    from bmc_jax.fitting import Experiment, FitSpec, fit, fit_joint, FreeParam
+    from bmc_jax.simulation import B0Spread, SimParams, initial_state
+    from bmc_jax.simulation.spread import node_params
+    ladder6, spec6, truth6, rng = scanner_workload(6, seed=10)
+    # tube 0 holds water only: pool of size 0 and rate 0 (its water is exactly that of no pool)
+    cat = lambda a, v: np.concatenate([np.asarray(v, np.asarray(a).dtype).reshape((1,) + np.shape(a)[1:]), a])
+    p6 = ladder6.params
+    p7 = SimParams(*[cat(getattr(p6, f), v) for f, v in zip(SimParams._fields, (
+        0.35, 0.5605, 1.0, [0.0], [0.0], [0.0], [1.0], [100.0], 0.0, 0.0, 0.0, 0.0, 1.0, 0, 2.8936, 267.5153, 0.0,
+        1.0))])
+    pool = tuple(range(1, 7))
+    events = ladder6.events
+    r2_factor, m_start, sigma = 1.1, 0.60, 3.0
+    truth_p = p7._replace(water_r2=p7.water_r2 * r2_factor)
+    spread = B0Spread(nodes=15)
+    _, w = spread.nodes_weights()
+    z_true = np.zeros((7, ladder6.n_offsets))
+    for i in range(7):
+        one = SimParams(*[np.asarray(f)[i] for f in truth_p])
+        nodes = node_params(one, sigma, spread)
+        m0 = initial_state(one, m_start=m_start)
+        m = _engine.simulate_states(nodes, events, m0, n_voxels=w.size, dtype=np.float64)
+        z_true[i] = np.tensordot(w, m[..., ladder6.mz_loc], axes=(0, 0))
+    z = z_true + 1e-3 * rng.standard_normal(z_true.shape)
+    m0_held = np.stack([initial_state(SimParams(*[np.asarray(f)[i] for f in p7]), m_start=0.557) for i in range(7)])
+    ladder7 = Experiment(p7, events, m0_held, ladder6.mz_loc)
+    lad6 = Experiment(p6, events, m0_held[1:], ladder6.mz_loc)
+    K, F, T2 = spec6.slots
+    Kp, Fp, T2p = (replace(q, samples=pool) for q in (K, F, T2))
+    R2F = FreeParam("r2w", "water_r2", None, 1.0, 0.5, 2.0, 1.0, scope="shared", mode="factor")
+    MS = FreeParam("m_start", "m_start", None, 1.0, 0.2, 1.0, 0.557, scope="shared")
+    SIG = FreeParam("sigma", "b0_spread", None, 10.0, 0.0, 8.0, 5.0, scope="shared")
+    spec = FitSpec([Kp, Fp, T2p, R2F, MS, SIG])
+    fit(lad6, spec6, z[1:], n_iter=2)                                  # warm: plan, graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = fit(lad6, spec6, z[1:], n_iter=60, n_per_block=82)
+    x0 = np.tile([q.x0 for q in spec.slots], (7, 1))
+    x0[list(pool), :3] = plain.x
+    costs = []
+    joint = fit_joint(ladder7, spec, z, n_iter=12, n_per_block=82, x0_phys=x0, spread=spread, max_chunk=3,
+                      callback=lambda it, c: costs.append(float(c)))
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    shared = [float(joint["r2w"][0]), float(joint["m_start"][0]), float(joint["sigma"][0])]
+    rel = np.abs(np.asarray(joint.x)[list(pool), :3] / np.asarray(truth6) - 1.0)
+    line = {"metric": "joint phantom fit wall time (reference tutorial 10 shape: 6-tube fit, 60 iterations, then "
+                      "fit_joint of 7 tubes with shared water R2 factor, M_start and 15-node B0 spread, 12 steps)",
+            "value": t, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * t,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": t / REF_JOINT_FIT_S,
+            "vs_baseline_note": "reference's published ~2 h on one RTX 3070 (examples/README.md:65)",
+            "dtype": "f64 simulate (f32 I/O), f64 host LM step",
+            "data": "synthetic Z (float64 model at seeded truths, shared R2 x1.1, M_start 0.60, 3 Hz spread, "
+                    "+ N(0, 1e-3)) on synthetic events of the converted scanner sequences' layout",
+            "config": {"workload": "joint QUESP fit", "tubes": 7, "spread_nodes": 15, "joint_steps": 12},
+            "shared_fitted": {"r2w_factor": shared[0], "m_start": shared[1], "sigma_hz": shared[2]},
+            "shared_truth": {"r2w_factor": r2_factor, "m_start": m_start, "sigma_hz": sigma},
+            "median_rel_err_k_f_r2p": np.median(rel, axis=0).tolist(), "cost_per_step": costs,
+            "median_rms": float(np.median(joint.rms))}
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(prm, m0, ev, nc, slots, budget_s=15.0):
     """The C oracle on this host's cores: whole spectra of the first voxels, as many as fit the budget."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -433,7 +508,7 @@ def main():
     ap.add_argument("--truths", type=int, default=1024)
     ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--image-voxels", type=int, default=1024)
-    ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven", "scanner"), default="pulsed",
+    ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven", "scanner", "joint"), default="pulsed",
                     help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
     ap.add_argument("--nn-train", type=int, default=2_000_000)
     ap.add_argument("--nn-test", type=int, default=262_144)
@@ -456,6 +531,10 @@ def main():
         _native.torch_cuda()
         bench_scanner(args)
         return
+    if args.workload == "joint":
+        _native.torch_cuda()
+        bench_joint(args)
+        return
     if args.workload == "design":
         _native.torch_cuda()
         bench_design(args)
@@ -475,7 +554,7 @@ def main():
                                       "MLP 244-512-512-256-5 GELU", "n_train": args.nn_train, "n_test": args.nn_test},
                "plain_lm_early_exit": nf["plain"], "plain_lm_all_iterations": nf["plain_fixed"], "nn_lm": nf["nn"],
                "speedup_vs_plain_early_exit": nf["speedup"], "speedup_vs_plain_all_iterations": nf["speedup_vs_fixed"],
-               "noisy_sigma_1e-3_rms_le_1.05_sigma": nz, "train_seconds": r["train_seconds"]}
+               "noisy_sigma_1e-3_rms_le_1.15_sigma": nz, "train_seconds": r["train_seconds"]}
         print(json.dumps(out), flush=True)
         return
 

@@ -328,7 +328,9 @@ def benchmark(n_train=2_000_000, n_test=262_144, epochs=30, n_iter=30, rms_tol=1
       plain       plain x0, LM with the same early exit;
       plain_fixed plain x0, every voxel runs all n_iter iterations (the reference's fit).
     Two held-out sets of the same draws: "noise_free" spectra at rms_tol, and "noisy" spectra (sigma
-    `noise`, seeded) at 1.05 sigma, where a fit reaches the noise floor. Everything after training stays
+    `noise`, seeded) at 1.15 sigma: the rms of a fit at the noise floor scatters by sigma / sqrt(2 O)
+    (4.5 % at O = 244 points), so 1.15 sigma is 3.3 such deviations above sigma, a band every fit that
+    reached the noise floor meets; 1.05 sigma would count 14 % of exact fits as failures. Everything after training stays
     on the GPU; each path is timed from the measured spectra in HBM. The median relative error of the
     fitted k and f against the truth (float64) is reported for every path."""
     torch = _native.torch_cuda()
@@ -341,7 +343,7 @@ def benchmark(n_train=2_000_000, n_test=262_144, epochs=30, n_iter=30, rms_tol=1
     Zn = Z + noise * torch.randn(Z.shape, generator=gen, device="cuda", dtype=torch.float32)
     xt = torch.as_tensor(x_true, device="cuda")
     out = dict(train_seconds=prop.train_seconds, final_train_loss=prop.loss[-1], n_test=n_test)
-    for name, Zs, tol in (("noise_free", Z, rms_tol), ("noisy", Zn, 1.05 * noise)):
+    for name, Zs, tol in (("noise_free", Z, rms_tol), ("noisy", Zn, 1.15 * noise)):
         out[name] = _compare_starts(prop, Zs, xt, n_iter, tol)
         out[name]["rms_tol"] = tol
     return out
