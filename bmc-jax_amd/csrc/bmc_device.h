@@ -465,48 +465,44 @@ __device__ void build_gen(const double* prm, const Ev& e, MtCache& mc, const int
   }
 }
 
-// Truncated Taylor series acc = T_m(X) v0 with tangents dacc = d(T_m(X) v0) (v0's own tangent dv0).
+// Truncated Taylor series acc = T_m(X) v0 with tangents dacc = d(T_m(X) v0) (v0's own tangent dv0),
+// by Horner's rule: w <- v0 + (X w) / k for k = m, ..., 1 from w = v0, one FMA per entry and term
+// where the power sum needs a scale and an add. The tangents follow the same recursion,
+// dw <- dv0 + (X dw + dX w) / k. The homogeneous coordinate of every iterate stays vaug0 (X's last row
+// is zero). acc / dacc may alias v0 / dv0 (in-place use): v0 is copied first.
 template <int NC, int PT>
 __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
                                        const double* dwp, int m, const double* v0, double vaug0,
                                        const double (*dv0)[3 * (1 + NC) + 1], double* acc,
                                        double (*dacc)[3 * (1 + NC) + 1]) {
   constexpr int N = 3 * (1 + NC) + 1;
-  double t[N], dtn[PT > 0 ? PT : 1][N];
+  double v[N], dv[PT > 0 ? PT : 1][N];
 #pragma unroll
-  for (int i = 0; i < N; i++) {
-    t[i] = v0[i];
-    acc[i] = v0[i];
-  }
+  for (int i = 0; i < N; i++) v[i] = v0[i];
 #pragma unroll
   for (int p = 0; p < PT; p++)
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-      dtn[p][i] = dv0 ? dv0[p][i] : 0.0;
-      dacc[p][i] = dtn[p][i];
-    }
-  double taug = vaug0;
-  for (int k = 1; k <= m; k++) {
+    for (int i = 0; i < N; i++) dv[p][i] = dv0 ? dv0[p][i] : 0.0;
+#pragma unroll
+  for (int i = 0; i < N; i++) acc[i] = v[i];
+#pragma unroll
+  for (int p = 0; p < PT; p++)
+#pragma unroll
+    for (int i = 0; i < N; i++) dacc[p][i] = dv[p][i];
+  for (int k = m; k >= 1; k--) {
     const double ik = kInv[k];
     double y[N];
-    apply_X<NC>(G, t, taug, y);
+    apply_X<NC>(G, acc, vaug0, y);
 #pragma unroll
     for (int p = 0; p < PT; p++) {
       double dy[N];
-      apply_X<NC>(G, dtn[p], 0.0, dy);
-      apply_dX<NC>(kind[p], kidx[p], tc[p], dwp, t, taug, dy);
+      apply_X<NC>(G, dacc[p], 0.0, dy);
+      apply_dX<NC>(kind[p], kidx[p], tc[p], dwp, acc, vaug0, dy);  // dX at the current iterate
 #pragma unroll
-      for (int i = 0; i < N; i++) {
-        dtn[p][i] = dy[i] * ik;
-        dacc[p][i] += dtn[p][i];
-      }
+      for (int i = 0; i < N; i++) dacc[p][i] = fma(dy[i], ik, dv[p][i]);
     }
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-      t[i] = y[i] * ik;
-      acc[i] += t[i];
-    }
-    taug = 0.0;
+    for (int i = 0; i < N; i++) acc[i] = fma(y[i], ik, v[i]);
   }
 }
 
