@@ -508,6 +508,8 @@ def main():
     ap.add_argument("--truths", type=int, default=1024)
     ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--image-voxels", type=int, default=1024)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 1 GiB device buffer (the reference point for scaling HBM counters)")
     ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven", "scanner", "joint"), default="pulsed",
                     help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
     ap.add_argument("--nn-train", type=int, default=2_000_000)
@@ -522,6 +524,15 @@ def main():
     import torch
     from bmc_jax import _engine, _flops, _native
     from bmc_jax.simulation.events import SimEvent
+
+    if args.calibrate_copy:
+        _native.torch_cuda()
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 4}), flush=True)
+        return
 
     if args.workload == "uneven":
         _native.torch_cuda()

@@ -16,7 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "bmc_jax", "_lib")
 LIB = os.path.join(OUT_DIR, "libbmc_hip.so")
-SOURCES = ["bmc_abi.hip", "bmc_sim.hip", "bmc_adj.hip", "bmc_uneven.hip", "bmc_lm.hip", "bmc_mlp.hip"]
+SOURCES = ["bmc_abi.hip", "bmc_sim.hip", "bmc_cwj.hip", "bmc_adj.hip", "bmc_uneven.hip", "bmc_lm.hip",
+           "bmc_mlp.hip"]
 ARCH = os.environ.get("BMC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -328,6 +328,23 @@ int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const d
   p.reset_after_adc = reset_after_adc ? 1 : 0;
   p.compressed = compress ? 1 : 0;
   p.has_keep = std::any_of(ops.begin(), ops.end(), [](const Op& op) { return op.kind == OP_KEEP; }) ? 1 : 0;
+  // single-event rows: at most one event with dt > 1e-9 (simulate.py:51), a delay or an RF block; no
+  // spoiler or gradient anywhere; after that event only zero-length events, any readout being the
+  // row's last event (so the row's last_M is the state the event leaves, simulate.py:105-132)
+  std::vector<int32_t> live(O, -1);
+  bool cw = T > 0;
+  for (int o = 0; o < O && cw; o++) {
+    for (int e = 0; e < T && cw; e++) {
+      const size_t k = (size_t)o * T + e;
+      if (type[k] == BMC_EV_GRADIENT || type[k] == BMC_EV_SPOILER) cw = false;
+      if (dt[k] > 1e-9) {
+        if (live[o] >= 0 || (type[k] != BMC_EV_DELAY && type[k] != BMC_EV_RF)) cw = false;
+        live[o] = e;
+      }
+      if (type[k] == BMC_EV_ADC && live[o] >= 0 && e != T - 1) cw = false;
+    }
+  }
+  p.cw_rows = cw ? 1 : 0;
   p.n_ops = (int64_t)ops.size();
   p.n_built = built;
   p.n_applied = applied;
@@ -343,6 +360,10 @@ int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const d
   if ((e = hipMalloc(&p.d_ops, sizeof(Op) * std::max<size_t>(ops.size(), 1))) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&p.d_rot, sizeof(double) * std::max<size_t>(rot.size(), 2))) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&p.d_row_ptr, sizeof(int32_t) * row_ptr.size())) != hipSuccess) return fail(e);
+  if (p.cw_rows) {
+    if ((e = hipMalloc(&p.d_live, sizeof(int32_t) * O)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpy(p.d_live, live.data(), sizeof(int32_t) * O, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+  }
   if (!ev.empty() && (e = hipMemcpy(p.d_ev, ev.data(), sizeof(Ev) * ev.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e);
   if (!ops.empty() && (e = hipMemcpy(p.d_ops, ops.data(), sizeof(Op) * ops.size(), hipMemcpyHostToDevice)) != hipSuccess)
@@ -364,6 +385,7 @@ int bmc_plan_destroy(bmc_plan* plan) {
   if (p.d_ops) (void)hipFree(p.d_ops);
   if (p.d_row_ptr) (void)hipFree(p.d_row_ptr);
   if (p.d_rot) (void)hipFree(p.d_rot);
+  if (p.d_live) (void)hipFree(p.d_live);
   delete plan;
   return BMC_OK;
 }
@@ -486,6 +508,7 @@ extern "C" int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int 
   a.PT = tangents_per_pass(want_tan);
   a.n_pass = want_tan ? (want_tan + a.PT - 1) / a.PT : 1;
   if (!want_tan) a.P = n_tangents;  // seeds unused
+  if (want_tan && use_cwj(plan->p, a)) return launch_cwj(n_cest, a, plan->p.d_live, (hipStream_t)stream);
   return launch_sim(n_cest, a, (hipStream_t)stream);
 }
 

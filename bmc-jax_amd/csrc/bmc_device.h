@@ -298,11 +298,12 @@ struct MtCache {
 };
 
 // Builds the generator of one event (times `sc` = dt 2^-s) and, for every tangent, its coefficients.
-// Returns the unscaled augmented 1-norm ||aug dt||_1 in l1.
+// Returns the unscaled augmented 1-norm ||aug dt||_1 in l1. mmax caps the Taylor degree (the
+// squaring count follows from it).
 template <int NC, int PT>
 __device__ void build_gen(const double* prm, const Ev& e, MtCache& mc, const int* kind, const int* kidx,
                           Gen<NC>& G, TanCo* tc, double& l1, double& sc_out, double nan_l1, int& m_out,
-                          int& s_out, bool& is_nan) {
+                          int& s_out, bool& is_nan, int mmax = kMMax) {
   using I = PrmIdx<NC>;
   const double r1w = prm[0], r2w = prm[1], kmt = prm[3], fmt = prm[4], dwmt = prm[5], r1mt = prm[6], r2mt = prm[7];
   const int ls = (int)prm[8];
@@ -355,15 +356,15 @@ __device__ void build_gen(const double* prm, const Ev& e, MtCache& mc, const int
   is_nan = !(l1 < nan_l1);
   // Taylor degree and squarings
   int m, s;
-  if (l1 <= kTheta[kMMax]) {
+  if (l1 <= kTheta[mmax]) {
     s = 0;
     m = 1;
-    while (m < kMMax && kTheta[m] < l1) m++;
+    while (m < mmax && kTheta[m] < l1) m++;
     if (l1 == 0.0) m = 1;
   } else {
-    s = (int)ceil(log2(l1 / kTheta[kMMax]));
+    s = (int)ceil(log2(l1 / kTheta[mmax]));
     if (s < 0) s = 0;
-    m = kMMax;
+    m = mmax;
     while (s > 0 && l1 * ldexp(1.0, -s) > kTheta[m]) s++;  // guard rounding of log2
   }
   if (is_nan) s = 0;

@@ -95,9 +95,11 @@ struct Plan {
   Op* d_ops = nullptr;
   int32_t* d_row_ptr = nullptr;
   double* d_rot = nullptr;  // [n_ops, 2]
+  int32_t* d_live = nullptr;  // [O] the one propagating event of a single-event row (-1: none)
   int64_t n_ops = 0, n_built = 0, n_applied = 0;
   int compressed = 1;
   int has_keep = 0;
+  int cw_rows = 0;  // every row propagates at most one event (a CW block) and ends at its readout
   std::vector<Op> h_ops;          // host copy of the program
   std::vector<int32_t> h_row_ptr; // host copy of the row offsets
 };
@@ -105,6 +107,11 @@ struct Plan {
 void set_error(const std::string& msg);
 int tan_kind_of_slot(int slot, int nc, int* idx);
 int launch_sim(int nc, const SimArgs& a, hipStream_t stream);
+// The Jacobian of single-event rows by reverse accumulation through the squarings (bmc_cwj.hip):
+// use_cwj says whether a tangent launch of this plan can take it (no M0 seeds, no trajectory; the
+// environment variable BMC_FORWARD_JACOBIAN=1 keeps the forward-mode kernel), launch_cwj runs it.
+bool use_cwj(const Plan& p, const SimArgs& a);
+int launch_cwj(int nc, const SimArgs& a, const int32_t* row_live, hipStream_t stream);
 // Tangents carried per simulate pass for P tangents (register budget of the kernel).
 int tangents_per_pass(int P);
 

@@ -377,6 +377,8 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   ac.zmeas = z_meas_dev;
   ac.s = st;
 
+  // single-event rows take the Jacobian by reverse accumulation through the squarings
+  const bool cwj = use_cwj(plan->p, sj);
   hipLaunchKernelGGL(init_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, st, V, lam0);
   BMC_HIP_CHECK(hipGetLastError());
   if ((flags & BMC_LM_START_CHECK) && (flags & BMC_LM_EARLY_EXIT)) {
@@ -397,7 +399,8 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
     hipLaunchKernelGGL(compact_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, (const int32_t*)st.jac_live, V,
                        (int32_t*)blj.p, cnt);
     BMC_HIP_CHECK(hipGetLastError());
-    if ((r = launch_sim(n_cest, sj, stream)) != BMC_OK) return r;
+    if ((r = cwj ? launch_cwj(n_cest, sj, plan->p.d_live, stream) : launch_sim(n_cest, sj, stream)) != BMC_OK)
+      return r;
     hipLaunchKernelGGL(normal_eq_kernel, dim3(V), dim3(64), 0, stream, ne);
     BMC_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(compact_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, (const int32_t*)st.active, V,

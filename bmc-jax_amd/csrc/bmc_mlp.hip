@@ -12,6 +12,8 @@
 //   A fragment: row l % 16, k = 4 (l / 16) .. +3       B fragment: column l % 16, k = 4 (l / 16) .. +3
 //   D fragment: column l % 16, rows 4 (l / 16) .. +3
 #include <math.h>
+
+#include <algorithm>
 #include <string.h>
 
 #include <string>
@@ -157,12 +159,29 @@ __global__ void gelu_back_kernel(const float* dA, const float* pre, long n, floa
   }
 }
 
-// db[n] = sum_b dZ[b, n]
-__global__ void bias_grad_kernel(const float* dZ, int B, int N, float* db) {
+// db[n] = sum_b dZ[b, n] in two passes with a fixed summation order (bitwise reproducible training):
+// partial sums over chunks of kBiasRows rows, 64 columns x 4 row lanes per block so that every row is
+// read in 256-byte runs, then one thread per column adds the chunks.
+constexpr int kBiasRows = 256;
+
+__global__ void __launch_bounds__(256) bias_grad_partial_kernel(const float* dZ, int B, int N, float* part) {
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int r1 = min(B, (int)(blockIdx.y + 1) * kBiasRows);
+  float s = 0.f;
+  if (c < N)
+    for (int r = blockIdx.y * kBiasRows + (threadIdx.x >> 6); r < r1; r += 4) s += dZ[(long)r * N + c];
+  __shared__ float sh[4][64];
+  sh[threadIdx.x >> 6][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (threadIdx.x < 64 && c < N)
+    part[(long)blockIdx.y * N + c] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+}
+
+__global__ void bias_grad_sum_kernel(const float* part, int chunks, int N, float* db) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   float s = 0.f;
-  for (int b = 0; b < B; b++) s += dZ[(long)b * N + n];
+  for (int k = 0; k < chunks; k++) s += part[(long)k * N + n];
   db[n] = s;
 }
 
@@ -209,6 +228,7 @@ struct bmc_mlp {
   std::vector<uint16_t*> act_bf, dact_bf;
   uint16_t* x_bf = nullptr;
   float* loss = nullptr;
+  float* bias_part = nullptr;  // [ceil(max_batch / kBiasRows), max width] partial bias gradients
   int step = 0;
 };
 
@@ -231,6 +251,7 @@ void mlp_free(bmc_mlp* h) {
   for (auto p : h->dact_bf) F(p);
   F(h->x_bf);
   F(h->loss);
+  F(h->bias_part);
   delete h;
 }
 
@@ -298,7 +319,11 @@ int backward(bmc_mlp* h, const float* T, int B, hipStream_t s) {
     gw.alpha = 1.f;
     int rc = gemm(gw, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(bias_grad_kernel, dim3((N + 255) / 256), dim3(256), 0, s, dZ, B, N, h->g + h->boff[l]);
+    const int chunks = (B + kBiasRows - 1) / kBiasRows;
+    hipLaunchKernelGGL(bias_grad_partial_kernel, dim3((N + 63) / 64, chunks), dim3(256), 0, s, dZ, B, N,
+                       h->bias_part);
+    hipLaunchKernelGGL(bias_grad_sum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, h->bias_part, chunks, N,
+                       h->g + h->boff[l]);
     BMC_HIP_CHECK(hipGetLastError());
     if (l > 0) {
       // dX_l [B, K] = dZ [B, N] W [N, K], into dact[l - 1]
@@ -349,7 +374,9 @@ int bmc_mlp_create(int n_layers, const int32_t* dims, int max_batch, uint64_t se
   bool ok = A((void**)&h->w, 4 * h->n_params) && A((void**)&h->g, 4 * h->n_params) &&
             A((void**)&h->m, 4 * h->n_params) && A((void**)&h->v, 4 * h->n_params) &&
             A((void**)&h->wbf, 2 * h->n_params) && A((void**)&h->x_bf, 2L * max_batch * dims[0]) &&
-            A((void**)&h->loss, 4);
+            A((void**)&h->loss, 4) &&
+            A((void**)&h->bias_part, 4L * ((max_batch + kBiasRows - 1) / kBiasRows) *
+                                         *std::max_element(dims + 1, dims + n_layers + 1));
   for (int l = 0; ok && l < n_layers; l++) {
     const size_t n = (size_t)max_batch * dims[l + 1];
     float *a, *b, *c;

@@ -75,8 +75,15 @@ def _uniform_sim(p, x_scaled):
     return m[0, :, 2 * (1 + nc)], J[0]
 
 
+@pytest.fixture
+def forward_jacobian(monkeypatch):
+    """The uniform path with its forward-mode Jacobian kernel, the arithmetic the uneven sweep shares
+    (single-event rows otherwise take the reverse kernel, equal to rounding: tests/test_cw_reverse.py)."""
+    monkeypatch.setenv("BMC_FORWARD_JACOBIAN", "1")
+
+
 @pytest.mark.parametrize("cfg", [(1, 1024, 1), (2, 256, 0), (1, 7, 0)])
-def test_simulate_matches_the_uniform_kernel(problems, cfg):
+def test_simulate_matches_the_uniform_kernel(problems, cfg, forward_jacobian):
     from bmc_jax.fitting.uneven import UnevenBatch
     b = UnevenBatch(problems, dtype=np.float64)
     b.configure(*cfg)
@@ -90,7 +97,7 @@ def test_simulate_matches_the_uniform_kernel(problems, cfg):
         np.testing.assert_allclose(b.jacobian_of(J, i), Ju, rtol=1e-12, atol=1e-14 * np.abs(Ju).max())
 
 
-def test_lm_matches_the_uniform_lm(problems):
+def test_lm_matches_the_uniform_lm(problems, forward_jacobian):
     from bmc_jax.fitting.independent import _lm_batched
     from bmc_jax.fitting.uneven import UnevenBatch
     from bmc_jax.simulation.voxels import tile_params
