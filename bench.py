@@ -29,9 +29,9 @@ sys.path[:0] = [os.path.join(ROOT, "bmc-jax_amd")]
 FREE = [("cest_k", 0), ("cest_f", 0), ("water_r1", None), ("water_r2", None), ("b0_inhom", None)]
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per voxel of one launch of this workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 1.146e12 B
-# per 65,536-voxel launch, almost all register-spill traffic.
-TRAFFIC_BYTES_PER_VOXEL = 1.146e12 / 65536
+# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 2.531e11 B
+# per 65,536-voxel launch (40.5 GB fetched, 212.6 GB written), almost all register-spill traffic.
+TRAFFIC_BYTES_PER_VOXEL = 2.531e11 / 65536
 
 
 def workload(n_vox, seed=2):
@@ -273,6 +273,8 @@ def bench_uneven(args):
     sweep_ms = e0.elapsed_time(e1)
     ops, rp = b.plan.program()
     ev_all = {k: np.asarray(getattr(b.plan_events, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    # single-event rows take the reverse-accumulation sweep (csrc/bmc_cwj.hip): count its own flops
+    live = None if os.environ.get("BMC_FORWARD_JACOBIAN", "0") not in ("", "0") else _flops.single_event_rows(ev_all)
     rng = np.random.default_rng(0)
     sample = rng.choice(b.n, size=min(64, b.n), replace=False)
     fl = 0.0
@@ -283,8 +285,11 @@ def bench_uneven(args):
         ev_i = {k: v[r0:r0 + nr] for k, v in ev_all.items()}
         from bmc_jax.fitting.params import write_params
         pw = write_params(p.params, p.spec.to_scaled(p.x0_phys), p.spec)
-        fl += float(_flops.executed_flops(pack(pw, 1), nc, ev_i, ops, rp[r0:r0 + nr + 1], spec_slots(p.spec, nc),
-                                          pt=best[0])[0])
+        if live is not None:
+            fl += float(_flops.reverse_flops(pack(pw, 1), nc, ev_i, spec_slots(p.spec, nc), live[r0:r0 + nr])[0])
+        else:
+            fl += float(_flops.executed_flops(pack(pw, 1), nc, ev_i, ops, rp[r0:r0 + nr + 1], spec_slots(p.spec, nc),
+                                              pt=best[0])[0])
     flops_sweep = fl / len(sample) * b.n
     achieved = flops_sweep / (sweep_ms * 1e-3) / 1e12
     # one bmc_lm_fit per problem (what a per-shape loop of `fit` calls launches), on a sample
@@ -308,7 +313,8 @@ def bench_uneven(args):
                                   "1-4 CEST pools + optional MT, 2-8 free values, 30 LM iterations in one hipGraph",
                       "problems": b.n, "rows": int(b.R), "values": int(b.n_values),
                       "autotuned": {"tangents_per_pass": best[0], "grid": best[1], "order": best[2]}},
-           "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (persistent sweep)",
+           "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (persistent "
+                        + ("reverse-accumulation Jacobian sweep)" if live is not None else "sweep)"),
                         "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "flops_per_launch": flops_sweep,
                         "avg_launch_ms": sweep_ms},

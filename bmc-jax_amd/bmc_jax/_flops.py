@@ -205,3 +205,58 @@ def adjoint_flops(prm, nc, events, slots, n_q=2):
             adj = adj + NA * m * term2 + s * sq2 + dots
         total += (2 * fwd + adj).sum(axis=1)
     return total
+
+
+def single_event_rows(events):
+    """The propagating event of every row when the table qualifies for the single-event Jacobian kernel
+    (csrc/bmc_cwj.hip; the rule of bmc_plan_create_ex): each row propagates at most one event (dt > 1e-9,
+    a delay or RF block), no spoiler or gradient anywhere, and a readout after that event is the row's
+    last event. Returns an int array [O] (-1: no propagating event) or None."""
+    ev = {k: np.asarray(getattr(events, k) if not isinstance(events, dict) else events[k]) for k in ("type", "dt")}
+    typ, dt = ev["type"], ev["dt"].astype(np.float64)
+    O, T = typ.shape
+    if T == 0 or np.any((typ == 2) | (typ == 4)):
+        return None
+    live = np.full(O, -1)
+    for o in range(O):
+        idx = np.flatnonzero(dt[o] > 1e-9)
+        if idx.size > 1 or (idx.size == 1 and typ[o, idx[0]] not in (0, 1)):
+            return None
+        if idx.size == 1:
+            live[o] = idx[0]
+            adc_after = np.flatnonzero(typ[o, idx[0]:] == 3) + idx[0]
+            if np.any(adc_after != T - 1):
+                return None
+    return live
+
+
+def reverse_flops(prm, nc, events, slots, live, chunk=2):
+    """Useful float64 flops per voxel of prm [V, S] of one single-event Jacobian launch (csrc/bmc_cwj.hip),
+    summed over rows, with the forward kernel's Taylor degree / squaring rule:
+      the Taylor columns of T            NA m (FX + 2N)
+      per squaring level (per lane)      E^2 and E^T G: 2 N^2 each; the rows of E^2 and G E^T: 2 N NA each
+      the state                          2 N NA
+      per chunk of T <= `chunk` tangents NA m (FX (1 + T) + dX + 2N (1 + T)), then 2 N NA + NA per tangent
+    live: single_event_rows(events)."""
+    ev = {k: np.asarray(getattr(events, k) if not isinstance(events, dict) else events[k])
+          for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    N = 3 * (1 + nc) + 1
+    NA = N + 1
+    FX = 29 * nc + 27
+    chunks = [slots[i:i + chunk] for i in range(0, len(slots), chunk)]
+    V = prm.shape[0]
+    total = np.zeros(V)
+    for o in np.flatnonzero(np.asarray(live) >= 0):
+        e = [int(live[o])]
+        ph = ev["rf_phase"][o, e].astype(np.float64)
+        l1 = _gen_l1(prm, nc, ev["dt"][o, e].astype(np.float64), ev["rf_amp"][o, e].astype(np.float64), np.cos(ph),
+                     np.sin(ph), ev["rf_freq"][o, e].astype(np.float64))
+        m, s = degree_and_squarings(l1)
+        m, s = m[:, 0], s[:, 0]
+        f = NA * m * (FX + 2 * N) + s * NA * (4 * N * N + 4 * N * NA) + 2 * N * NA
+        for ch in chunks:
+            T = len(ch)
+            fdx = sum(dx_flops(sl, nc) for sl in ch)
+            f = f + NA * m * (FX * (1 + T) + fdx + 2 * N * (1 + T)) + T * (2 * N * NA + NA)
+        total += f
+    return total

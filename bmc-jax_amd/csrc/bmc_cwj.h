@@ -1,0 +1,144 @@
+// The single-event row and its Jacobian by reverse accumulation through the squarings, shared by the
+// simulate / LM kernels (bmc_cwj.hip) and the uneven batch (bmc_uneven.hip). Derivation in bmc_cwj.hip.
+#pragma once
+
+#include "bmc_device.h"
+
+namespace bmc {
+
+constexpr int kCwjChunk = 2;  // tangents per Taylor pass
+
+// One level: G <- G E^T + E^T G, then E <- E E (columns R, rows Er), lane j's share.
+template <int NC, int TEAM>
+__device__ __forceinline__ void cwj_level(double* R, double* Er, double* Gc, int j) {
+  constexpr int N = 3 * (1 + NC) + 1, NA = N + 1;
+  double o[N], ner[NA], etg[N], nge[N];
+  const bool aff = (j == N);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    o[i] = aff ? R[i] : 0.0;  // the E[i][N] E[N][j] term (E[N][j] = [j == N])
+    etg[i] = 0.0;
+    nge[i] = 0.0;
+  }
+#pragma unroll
+  for (int l = 0; l < NA; l++) ner[l] = 0.0;
+  ner[N] = Er[N];  // E[j][N] E[N][N]
+  static_for<0, NA>([&](auto lc) {
+    constexpr int l = decltype(lc)::value;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      const double b = bcast<TEAM, l>(R[i]);  // E[i][l]
+      if constexpr (l < N) {
+        o[i] = fma(b, R[l], o[i]);        // (E E)[i][j]   += E[i][l] E[l][j]
+        etg[l] = fma(b, Gc[i], etg[l]);   // (E^T G)[l][j] += E[i][l] G[i][j]
+      }
+      ner[l] = fma(Er[i], b, ner[l]);     // (E E)[j][l]   += E[j][i] E[i][l]
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      const double g = bcast<TEAM, l>(Gc[i]);  // G[i][l]
+      nge[i] = fma(g, Er[l], nge[i]);          // (G E^T)[i][j] += G[i][l] E[j][l]
+    }
+  });
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    R[i] = o[i];
+    Gc[i] = etg[i] + nge[i];
+  }
+#pragma unroll
+  for (int l = 0; l < NA; l++) Er[l] = ner[l];
+}
+
+// One team's row: the state M = E v (every lane, team reductions) and, for the P tangents of kinds
+// kind[] / kidx[] (indexable by tangent), dZ/dp for Z = M[jc]. e_live is the row's propagating event
+// (null: none, E = I). store_state(M, isn) and store_jac(p, dZ/dp, isn) are called by lane 0 only.
+template <int NC, int TEAM, typename KindAt, typename StoreState, typename StoreJac>
+__device__ __forceinline__ void cwj_row(const double* prm, const double* m0, const double* dwp, const Ev* e_live,
+                                        int P, KindAt kind_at, int jc, double nan_l1, int lane, bool want_jac,
+                                        StoreState&& store_state, StoreJac&& store_jac) {
+  constexpr int NT = 1 + NC, N = 3 * NT + 1, NA = N + 1;
+  double vj = (lane == N) ? 1.0 : 0.0;  // lane's entry of the augmented start vector
+#pragma unroll
+  for (int i = 0; i < N; i++)
+    if (lane == i) vj = m0[i];
+  double R[N], Er[NA], Gc[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    R[i] = (i == lane) ? 1.0 : 0.0;
+    Gc[i] = (i == jc) ? vj : 0.0;  // G_1 = e_z v^T
+  }
+  int m = 0, s = 0;
+  bool isn = false;
+  MtCache mc;
+  mc.fr = __builtin_nan("");
+  mc.s = MtShape{0.0, 0.0, 0.0, 0.0};
+  Ev e{};
+  if (e_live) {
+    e = *e_live;
+    Gen<NC> G;
+    int k0[1] = {TK_NONE}, i0[1] = {0};
+    TanCo tc0[1];
+    double l1, sc;
+    build_gen<NC, 0>(prm, e, mc, k0, i0, G, tc0, l1, sc, nan_l1, m, s, isn, kMMax);
+    double dR0[1][N];
+    taylor<NC, 0>(G, tc0, k0, i0, dwp, m, R, (lane == N) ? 1.0 : 0.0, nullptr, R, dR0);  // column lane of T
+    // rows of T: Er[l] = T[lane][l]
+    static_for<0, NA>([&](auto lc) {
+      constexpr int l = decltype(lc)::value;
+      double x = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const double b = bcast<TEAM, l>(R[i]);
+        if (i == lane) x = b;
+      }
+      Er[l] = x;
+    });
+    if (lane == N) Er[N] = 1.0;
+    for (int q = 0; q < s; q++) cwj_level<NC, TEAM>(R, Er, Gc, lane);
+  }
+  double M[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) M[i] = team_sum<TEAM>(R[i] * vj);
+  if (lane == 0) store_state(M, isn);
+  if (!want_jac) return;
+  // tangents: dZ/dp = <dT_p, G>, dT_p by forward-mode Horner, kCwjChunk tangents per pass
+  for (int q0 = 0; q0 < P; q0 += kCwjChunk) {
+    int kind[kCwjChunk], kidx[kCwjChunk];
+#pragma unroll
+    for (int p = 0; p < kCwjChunk; p++) {
+      kind[p] = TK_NONE;
+      kidx[p] = 0;
+      if (q0 + p < P) kind_at(q0 + p, kind[p], kidx[p]);
+    }
+    double g[kCwjChunk];
+    if (e_live) {
+      TanCo tc[kCwjChunk];
+      double l1, sc;
+      int mq, sq;
+      bool isq;
+      Gen<NC> Gq;
+      build_gen<NC, kCwjChunk>(prm, e, mc, kind, kidx, Gq, tc, l1, sc, nan_l1, mq, sq, isq, kMMax);
+      double T[N], dT[kCwjChunk][N];
+#pragma unroll
+      for (int i = 0; i < N; i++) T[i] = (i == lane) ? 1.0 : 0.0;
+      taylor<NC, kCwjChunk>(Gq, tc, kind, kidx, dwp, mq, T, (lane == N) ? 1.0 : 0.0, nullptr, T, dT);
+#pragma unroll
+      for (int p = 0; p < kCwjChunk; p++) {
+        double d = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; i++) d = fma(dT[p][i], Gc[i], d);
+        g[p] = team_sum<TEAM>(lane < NA ? d : 0.0);
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < kCwjChunk; p++) g[p] = 0.0;  // no propagating event: Z = M0[jc]
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int p = 0; p < kCwjChunk; p++)
+        if (q0 + p < P) store_jac(q0 + p, g[p], isn);
+    }
+  }
+}
+
+}  // namespace bmc

@@ -22,57 +22,13 @@
 
 #include <algorithm>
 
-#include "bmc_device.h"
+#include "bmc_cwj.h"
 
 namespace bmc {
-
-constexpr int kCwjMMax = kMMax;
-constexpr int kCwjChunk = 2;  // tangents per Taylor pass
 
 #ifndef BMC_CWJ_WAVES_PER_SIMD
 #define BMC_CWJ_WAVES_PER_SIMD 2
 #endif
-
-// One level: G <- G E^T + E^T G, then E <- E E (columns R, rows Er), lane j's share.
-template <int NC, int TEAM>
-__device__ __forceinline__ void cwj_level(double* R, double* Er, double* Gc, int j) {
-  constexpr int N = 3 * (1 + NC) + 1, NA = N + 1;
-  double o[N], ner[NA], etg[N], nge[N];
-  const bool aff = (j == N);
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    o[i] = aff ? R[i] : 0.0;  // the E[i][N] E[N][j] term (E[N][j] = [j == N])
-    etg[i] = 0.0;
-    nge[i] = 0.0;
-  }
-#pragma unroll
-  for (int l = 0; l < NA; l++) ner[l] = 0.0;
-  ner[N] = Er[N];  // E[j][N] E[N][N]
-  static_for<0, NA>([&](auto lc) {
-    constexpr int l = decltype(lc)::value;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      const double b = bcast<TEAM, l>(R[i]);  // E[i][l]
-      if constexpr (l < N) {
-        o[i] = fma(b, R[l], o[i]);        // (E E)[i][j]   += E[i][l] E[l][j]
-        etg[l] = fma(b, Gc[i], etg[l]);   // (E^T G)[l][j] += E[i][l] G[i][j]
-      }
-      ner[l] = fma(Er[i], b, ner[l]);     // (E E)[j][l]   += E[j][i] E[i][l]
-    }
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-      const double g = bcast<TEAM, l>(Gc[i]);  // G[i][l]
-      nge[i] = fma(g, Er[l], nge[i]);          // (G E^T)[i][j] += G[i][l] E[j][l]
-    }
-  });
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    R[i] = o[i];
-    Gc[i] = etg[i] + nge[i];
-  }
-#pragma unroll
-  for (int l = 0; l < NA; l++) Er[l] = ner[l];
-}
 
 // One team's (voxel, row). Every thread of the block calls it (block barriers before the first exit).
 template <int NC, int TEAM, bool LISTED>
@@ -122,104 +78,30 @@ __device__ __forceinline__ void cwj_team(const SimArgs& a, const int32_t* row_li
   double dwp[NC > 0 ? NC : 1];
 #pragma unroll
   for (int i = 0; i < NC; i++) dwp[i] = prm[13 + 2 * NC + i];
-  const double* m0 = sh_m0[tib];
-  double vj = (lane == N) ? 1.0 : 0.0;  // lane's entry of the augmented start vector
-#pragma unroll
-  for (int i = 0; i < N; i++)
-    if (lane == i) vj = m0[i];
-  double R[N], Er[NA], Gc[N];
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    R[i] = (i == lane) ? 1.0 : 0.0;
-    Gc[i] = (i == a.jc) ? vj : 0.0;  // G_1 = e_z v^T
-  }
   const int le = row_live[o];
-  Gen<NC> G;
-  int m = 0, s = 0;
-  bool isn = false;
-  MtCache mc;
-  mc.fr = __builtin_nan("");
-  mc.s = MtShape{0.0, 0.0, 0.0, 0.0};
-  Ev e{};
-  if (le >= 0) {
-    e = a.ev[(long)o * a.T + le];
-    int k0[1] = {TK_NONE}, i0[1] = {0};
-    TanCo tc0[1];
-    double l1, sc;
-    build_gen<NC, 0>(prm, e, mc, k0, i0, G, tc0, l1, sc, a.nan_l1, m, s, isn, kCwjMMax);
-    double dR0[1][N];
-    taylor<NC, 0>(G, tc0, k0, i0, dwp, m, R, (lane == N) ? 1.0 : 0.0, nullptr, R, dR0);  // column lane of T
-    // rows of T: Er[l] = T[lane][l]
-    static_for<0, NA>([&](auto lc) {
-      constexpr int l = decltype(lc)::value;
-      double x = 0.0;
-#pragma unroll
-      for (int i = 0; i < N; i++) {
-        const double b = bcast<TEAM, l>(R[i]);
-        if (i == lane) x = b;
-      }
-      Er[l] = x;
-    });
-    if (lane == N) Er[N] = 1.0;
-    for (int q = 0; q < s; q++) cwj_level<NC, TEAM>(R, Er, Gc, lane);
-  }
-  // the final state M = E v (team reductions: every lane takes part)
-  double M[N];
-#pragma unroll
-  for (int i = 0; i < N; i++) M[i] = team_sum<TEAM>(R[i] * vj);
   const long vo = (long)v * a.O + o;
-  if (!dead && lane == 0) {
-    if (a.m_out)
-      for (int i = 0; i < N; i++) store_real(a.m_out, vo * N + i, isn ? __builtin_nan("") : M[i], a.dtype);
-    if (a.z_out) {
-      double z = 0.0;
+  cwj_row<NC, TEAM>(
+      prm, sh_m0[tib], dwp, le >= 0 ? a.ev + (long)o * a.T + le : nullptr, a.P,
+      [&](int p, int& k, int& ix) {
+        k = sh_tk[p];
+        ix = sh_ti[p];
+      },
+      a.jc, a.nan_l1, lane, a.jac_out != nullptr,
+      [&](const double* M, bool isn) {
+        if (dead) return;
+        if (a.m_out)
+          for (int i = 0; i < N; i++) store_real(a.m_out, vo * N + i, isn ? __builtin_nan("") : M[i], a.dtype);
+        if (a.z_out) {
+          double z = 0.0;
 #pragma unroll
-      for (int i = 0; i < N; i++)
-        if (i == a.jc) z = M[i];
-      a.z_out[vo] = isn ? __builtin_nan("") : z;
-    }
-  }
-  if (!a.jac_out) return;
-  // tangents: dZ/dp = <dT_p, G>, dT_p by forward-mode Horner, kCwjChunk tangents per pass
-  for (int q0 = 0; q0 < a.P; q0 += kCwjChunk) {
-    int kind[kCwjChunk], kidx[kCwjChunk];
-#pragma unroll
-    for (int p = 0; p < kCwjChunk; p++) {
-      kind[p] = (q0 + p < a.P) ? sh_tk[q0 + p] : TK_NONE;
-      kidx[p] = (q0 + p < a.P) ? sh_ti[q0 + p] : 0;
-    }
-    double g[kCwjChunk];
-    if (le >= 0) {
-      TanCo tc[kCwjChunk];
-      double l1, sc;
-      int mq, sq;
-      bool isq;
-      Gen<NC> Gq;
-      build_gen<NC, kCwjChunk>(prm, e, mc, kind, kidx, Gq, tc, l1, sc, a.nan_l1, mq, sq, isq, kCwjMMax);
-      double T[N], dT[kCwjChunk][N];
-#pragma unroll
-      for (int i = 0; i < N; i++) T[i] = (i == lane) ? 1.0 : 0.0;
-      taylor<NC, kCwjChunk>(Gq, tc, kind, kidx, dwp, mq, T, (lane == N) ? 1.0 : 0.0, nullptr, T, dT);
-#pragma unroll
-      for (int p = 0; p < kCwjChunk; p++) {
-        double d = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; i++) d = fma(dT[p][i], Gc[i], d);
-        g[p] = team_sum<TEAM>(lane < NA ? d : 0.0);
-      }
-    } else {
-#pragma unroll
-      for (int p = 0; p < kCwjChunk; p++) g[p] = 0.0;  // no propagating event: Z = M0[jc]
-    }
-    if (!dead && lane == 0) {
-#pragma unroll
-      for (int p = 0; p < kCwjChunk; p++) {
-        const int gp = q0 + p;
-        if (gp < a.P)
-          store_real(a.jac_out, vo * a.P + gp, isn ? __builtin_nan("") : g[p] * a.t_scale[gp], a.jac_dtype);
-      }
-    }
-  }
+          for (int i = 0; i < N; i++)
+            if (i == a.jc) z = M[i];
+          a.z_out[vo] = isn ? __builtin_nan("") : z;
+        }
+      },
+      [&](int gp, double val, bool isn) {
+        if (!dead) store_real(a.jac_out, vo * a.P + gp, isn ? __builtin_nan("") : val * a.t_scale[gp], a.jac_dtype);
+      });
 }
 
 template <int NC, int TEAM>
@@ -275,10 +157,14 @@ static int launch_cwj_nc(const SimArgs& a, const int32_t* row_live, hipStream_t 
   return BMC_OK;
 }
 
+bool forward_jacobian_forced() {
+  const char* f = getenv("BMC_FORWARD_JACOBIAN");
+  return f && f[0] && f[0] != '0';
+}
+
 bool use_cwj(const Plan& p, const SimArgs& a) {
   if (!p.cw_rows || !p.d_live || a.dm0 || a.traj || a.P <= 0 || !a.jac_out) return false;
-  const char* f = getenv("BMC_FORWARD_JACOBIAN");
-  return !(f && f[0] && f[0] != '0');
+  return !forward_jacobian_forced();
 }
 
 int launch_cwj(int nc, const SimArgs& a, const int32_t* row_live, hipStream_t stream) {

@@ -111,6 +111,7 @@ int launch_sim(int nc, const SimArgs& a, hipStream_t stream);
 // use_cwj says whether a tangent launch of this plan can take it (no M0 seeds, no trajectory; the
 // environment variable BMC_FORWARD_JACOBIAN=1 keeps the forward-mode kernel), launch_cwj runs it.
 bool use_cwj(const Plan& p, const SimArgs& a);
+bool forward_jacobian_forced();
 int launch_cwj(int nc, const SimArgs& a, const int32_t* row_live, hipStream_t stream);
 // Tangents carried per simulate pass for P tangents (register budget of the kernel).
 int tangents_per_pass(int P);

@@ -18,7 +18,7 @@
 #include <numeric>
 #include <vector>
 
-#include "bmc_device.h"
+#include "bmc_cwj.h"
 
 namespace bmc {
 
@@ -59,6 +59,7 @@ struct USimArgs {
   double* jac_out;  // [sum n_rows P] or null
   const int32_t* active;
   double nan_l1;
+  const int32_t* row_live;  // [R] or null: the Jacobian sweep of single-event rows by reverse accumulation
 };
 
 constexpr int kUMaxN = 3 * (1 + 4) + 1;
@@ -135,6 +136,69 @@ __device__ __forceinline__ void u_chunk(const USimArgs& a, int2 ch, double (*sh_
   }
 }
 
+// One chunk of (problem, row) items of single-event rows: state and Jacobian by reverse accumulation
+// through the squarings (cwj_row, bmc_cwj.h), every tangent of the problem in one item.
+template <int NC, int TEAM>
+__device__ __forceinline__ void u_chunk_cwj(const USimArgs& a, int2 ch, double (*sh_prm)[PrmIdx<4>::size],
+                                            double (*sh_start)[(1 + 3) * kUMaxN]) {
+  constexpr int NT = 1 + NC, N = 3 * NT + 1;
+  using I = PrmIdx<NC>;
+  const int lane = threadIdx.x % TEAM;
+  const int tib = threadIdx.x / TEAM;
+  const bool has = tib < ch.y;
+  const UItem it = a.items[ch.x + (has ? tib : 0)];
+  const UProb pr = a.probs[it.prob];
+  const bool dead = !has || (a.active && !a.active[it.prob]);
+  double* prm = sh_prm[tib];
+  for (int i = lane; i < I::stride; i += TEAM) {
+    double val = load_real(a.prm[NC - 1], (long)pr.local * I::stride + i, a.dtype);
+    for (int w = 0; w < pr.P; w++)
+      if (a.t_slot[pr.x_off + w] == i) {  // write_params (fitting/params.py:100-112)
+        const double xv = a.xw[pr.x_off + w] * a.t_scale[pr.x_off + w];
+        val = (a.dtype == BMC_F32) ? (double)(float)xv : xv;
+      }
+    prm[i] = val;
+  }
+  double* st0 = sh_start[tib];
+  for (int i = lane; i < N; i += TEAM) st0[i] = load_real(a.m0[NC - 1], (long)pr.local * N + i, a.dtype);
+  __syncthreads();
+  if (lane == 0) {
+    double L = prm[10] * 1e6 * prm[9];
+    double ks = 0.0;
+    for (int i = 0; i < NC; i++) ks += prm[13 + i] * prm[13 + NC + i];
+    prm[I::L] = L;
+    prm[I::KS] = ks;
+    prm[I::KWMT] = prm[3] * prm[4];
+    prm[I::DW0] = L * prm[11] * 1e-6;
+  }
+  __syncthreads();
+  if (__ballot(!dead) == 0) return;  // wave-uniform
+  double dwp[NC];
+#pragma unroll
+  for (int i = 0; i < NC; i++) dwp[i] = prm[13 + 2 * NC + i];
+  const int le = a.row_live[it.row];
+  cwj_row<NC, TEAM>(
+      prm, st0, dwp, le >= 0 ? a.ev + (long)it.row * a.T + le : nullptr, pr.P,
+      [&](int p, int& k, int& ix) {
+        k = a.t_kind[pr.x_off + p];
+        ix = a.t_idx[pr.x_off + p];
+      },
+      pr.jc, a.nan_l1, lane, a.jac_out != nullptr,
+      [&](const double* M, bool isn) {
+        if (dead) return;
+        double z = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+          if (i == pr.jc) z = M[i];
+        a.z_out[it.row] = isn ? __builtin_nan("") : z;
+      },
+      [&](int gp, double val, bool isn) {
+        if (!dead)
+          a.jac_out[pr.j_off + (long)(it.row - pr.row0) * pr.P + gp] =
+              isn ? __builtin_nan("") : val * a.t_scale[pr.x_off + gp];
+      });
+}
+
 #ifndef BMC_USIM_WAVES_PER_SIMD
 #define BMC_USIM_WAVES_PER_SIMD 1
 #endif
@@ -167,6 +231,37 @@ __global__ void __launch_bounds__(64, BMC_USIM_WAVES_PER_SIMD) usim_kernel(USimA
         break;
       default:
         u_chunk<4, PT, 32>(a, ch, sh_prm, sh_state, sh_start);
+        break;
+    }
+    __syncthreads();
+  }
+}
+
+// The Jacobian sweep of single-event rows: persistent over the one-item-per-row chunks.
+__global__ void __launch_bounds__(64, BMC_USIM_WAVES_PER_SIMD) ucwj_kernel(USimArgs a) {
+  __shared__ double sh_prm[8][PrmIdx<4>::size];
+  __shared__ double sh_start[8][(1 + 3) * kUMaxN];
+  __shared__ int sh_chunk;
+  for (;;) {
+    if (threadIdx.x == 0) sh_chunk = atomicAdd(a.counter, 1);
+    __syncthreads();
+    const int c = sh_chunk;
+    __syncthreads();
+    if (c >= a.n_chunks) break;  // every wave reaches this: the grid drains when the list is done
+    const int2 ch = a.chunks[c];
+    const int cls = a.probs[a.items[ch.x].prob].cls;
+    switch (cls) {
+      case 0:
+        u_chunk_cwj<1, 8>(a, ch, sh_prm, sh_start);
+        break;
+      case 1:
+        u_chunk_cwj<2, 16>(a, ch, sh_prm, sh_start);
+        break;
+      case 2:
+        u_chunk_cwj<3, 16>(a, ch, sh_prm, sh_start);
+        break;
+      default:
+        u_chunk_cwj<4, 32>(a, ch, sh_prm, sh_start);
         break;
     }
     __syncthreads();
@@ -429,6 +524,8 @@ USimArgs sim_args(const bmc_ubatch* b, bool tangents, const double* xw, double* 
   a.ops = p.d_ops;
   a.row_ptr = p.d_row_ptr;
   a.T = p.T;
+  // single-event rows: the Jacobian sweep takes one item per row and every tangent at once
+  const bool reverse = tangents && p.cw_rows && p.d_live && !forward_jacobian_forced();
   a.reset_after_adc = p.reset_after_adc;
   a.dtype = b->dtype;
   a.PT = tangents ? b->PT : 0;
@@ -442,9 +539,10 @@ USimArgs sim_args(const bmc_ubatch* b, bool tangents, const double* xw, double* 
   a.t_slot = b->d_slot;
   a.t_scale = b->d_scale;
   a.xw = xw;
-  a.items = tangents ? b->d_items_j : b->d_items_p;
-  a.chunks = tangents ? b->d_chunks_j : b->d_chunks_p;
-  a.n_chunks = tangents ? b->n_chunks_j : b->n_chunks_p;
+  a.items = tangents && !reverse ? b->d_items_j : b->d_items_p;
+  a.chunks = tangents && !reverse ? b->d_chunks_j : b->d_chunks_p;
+  a.n_chunks = tangents && !reverse ? b->n_chunks_j : b->n_chunks_p;
+  a.row_live = reverse ? p.d_live : nullptr;
   a.counter = b->d_counter;
   a.z_out = z_out;
   a.jac_out = tangents ? jac_out : nullptr;
@@ -457,6 +555,11 @@ int launch_usim(const bmc_ubatch* b, const USimArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, a.counter, 0);
   BMC_HIP_CHECK(hipGetLastError());
   const dim3 grid(std::max(1, std::min(b->grid, std::max(a.n_chunks, 1)))), block(64);
+  if (a.row_live) {
+    hipLaunchKernelGGL(ucwj_kernel, grid, block, 0, st, a);
+    BMC_HIP_CHECK(hipGetLastError());
+    return BMC_OK;
+  }
   switch (a.PT) {
     case 0:
       hipLaunchKernelGGL(usim_kernel<0>, grid, block, 0, st, a);

@@ -110,3 +110,26 @@ def test_reverse_jacobian_matches_forward_mode(nc):
     for p in range(len(slots)):
         scale = max(np.abs(Jf[..., p]).max(), 1e-30)
         assert np.max(np.abs(Jr[..., p] - Jf[..., p])) / scale < 1e-9, names[p]
+
+
+def test_single_event_rule():
+    """Which event tables take the reverse kernel: CW rows (front padding allowed), not pulsed trains,
+    spoilers, or a readout before the row's end."""
+    from bmc_jax import _flops
+    from bmc_jax.sequences import cw_saturation, pulsed_train
+    from bmc_jax.simulation.events import pad_events
+    ev = pad_events(cw_saturation(np.linspace(-3, 3, 5), 1.0, 2.0, 3.0), 4)   # [pad, pad, RF, ADC]
+    np.testing.assert_array_equal(_flops.single_event_rows(ev), [2] * 5)
+    assert _flops.single_event_rows(pulsed_train(np.array([1.0]), n_pulses=2, n_samples=3)) is None
+    t = np.array(ev.type, copy=True)
+    t[0, 0] = 4                                    # a zero-length spoiler still zeroes the transverse state
+    assert _flops.single_event_rows(ev._replace(type=t)) is None
+    t = np.array(ev.type, copy=True)
+    t[1, 3] = 0                                    # the readout no longer last ...
+    t[1, 1] = 3                                    # ... and one before the block: still fine (reset to M0)
+    d = np.array(ev.dt, copy=True)
+    assert _flops.single_event_rows(ev._replace(type=t, dt=d)) is not None
+    t2 = np.array(ev.type, copy=True)
+    t2 = np.concatenate([t2, np.zeros((5, 1), t2.dtype)], axis=1)   # an event after the readout
+    ev2 = ev._replace(type=t2, dt=np.concatenate([d, np.zeros((5, 1))], axis=1))
+    assert _flops.single_event_rows(ev2) is None
