@@ -133,3 +133,42 @@ def test_single_event_rule():
     t2 = np.concatenate([t2, np.zeros((5, 1), t2.dtype)], axis=1)   # an event after the readout
     ev2 = ev._replace(type=t2, dt=np.concatenate([d, np.zeros((5, 1))], axis=1))
     assert _flops.single_event_rows(ev2) is None
+
+
+@pytest.mark.gpu
+def test_reverse_kernel_edge_rows():
+    """Rows with no propagating event (Z = M0[jc], zero Jacobian), a row past the reference's NaN rule
+    (squaring count above 16: NaN state and Jacobian) and ordinary CW rows, reverse against forward."""
+    import torch
+    from bmc_jax import _native as nat
+    nat.torch_cuda()
+    p = O.make_params(1.3, 0.075, pools=[dict(f=9.01e-4, k=50.0, dw=3.5, t1=1.3, t2=0.1)], b0=3.0)
+    ev = O.cw_saturation(np.linspace(-6, 6, 6), b1_uT=1.0, tsat_s=2.0, b0_T=3.0)
+    ev = {k: np.array(v, copy=True) for k, v in ev.items()}
+    ev["dt"][1, 0] = 0.0            # no propagating event
+    ev["dt"][2, 0] = 5e3            # 1-norm far beyond theta_ref 2^17: the reference gives NaN
+    slots = [OC.slot_index("cest_k", 0, 1), OC.slot_index("cest_f", 0, 1), OC.slot_index("water_r2", None, 1)]
+    plan = nat.Plan(ev["type"], ev["dt"], ev["rf_amp"], ev["rf_phase"], ev["rf_freq"])
+    recs = [p, p]
+    prm = nat.device_array(pack_many(recs), torch.float64)
+    m0s = np.stack([O.initial_state(q) for q in recs]) * np.array([[1.0], [0.6]])
+    m0 = nat.device_array(m0s, torch.float64)
+    out = {}
+    for mode in ("reverse", "forward"):
+        if mode == "forward":
+            os.environ["BMC_FORWARD_JACOBIAN"] = "1"
+        try:
+            m, J = nat.simulate(plan, prm, 1, nat.BMC_F64, m0, slots=slots, scales=[1.0] * 3, jc=4)
+            torch.cuda.synchronize()
+        finally:
+            os.environ.pop("BMC_FORWARD_JACOBIAN", None)
+        out[mode] = (m.cpu().numpy(), J.cpu().numpy())
+    (mr, Jr), (mf, Jf) = out["reverse"], out["forward"]
+    np.testing.assert_array_equal(np.isnan(mr), np.isnan(mf))
+    np.testing.assert_array_equal(np.isnan(Jr), np.isnan(Jf))
+    assert np.all(np.isnan(mr[:, 2])) and np.all(np.isnan(Jr[:, 2]))
+    np.testing.assert_array_equal(mr[:, 1], m0s)           # nothing propagates: the start state
+    assert np.all(Jr[:, 1] == 0.0)
+    ok = [0, 3, 4, 5]
+    assert np.max(np.abs(mr[:, ok] - mf[:, ok])) < 1e-12
+    assert np.max(np.abs(Jr[:, ok] - Jf[:, ok])) < 1e-9 * np.abs(Jf[:, ok]).max()
