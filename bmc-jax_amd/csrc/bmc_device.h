@@ -44,10 +44,43 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+// Team geometry. Teams of 8, 16 and 32 lanes tile a wave. 12-lane teams (augmented size 9..12, two CEST
+// pools) sit five to a wave, at lanes [0, 12), [12, 24), [24, 36), [36, 48) and [48, 64), the last
+// with four more idle lanes: 60 of 64 lanes carry a column instead of 44 of 64 with 16-lane teams.
+template <int TEAM>
+__host__ __device__ constexpr int teams_per_block() {
+  return TEAM == 12 ? 20 : 256 / TEAM;  // 256-thread blocks
+}
+template <int TEAM>
+__device__ __forceinline__ int team_base_in_wave() {  // the team's first lane within its wave
+  if constexpr (TEAM == 12) {
+    const int wl = threadIdx.x & 63;
+    return (wl >= 48 ? 4 : wl / 12) * 12;
+  } else {
+    return (threadIdx.x & 63) / TEAM * TEAM;
+  }
+}
+template <int TEAM>
+__device__ __forceinline__ int team_lane() {
+  if constexpr (TEAM == 12)
+    return (threadIdx.x & 63) - team_base_in_wave<12>();
+  else
+    return threadIdx.x % TEAM;
+}
+template <int TEAM>
+__device__ __forceinline__ int team_index() {  // the team within its 256-thread block
+  if constexpr (TEAM == 12)
+    return (threadIdx.x >> 6) * 5 + team_base_in_wave<12>() / 12;
+  else
+    return threadIdx.x / TEAM;
+}
+
 // Broadcast lane K of the team to every lane of the team.
 template <int TEAM, int K>
 __device__ __forceinline__ double bcast(double x) {
-  if constexpr (TEAM == 16) {
+  if constexpr (TEAM == 12) {
+    return __shfl(x, team_base_in_wave<12>() + K, 64);
+  } else if constexpr (TEAM == 16) {
     return __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);  // row_newbcast:K
   } else if constexpr (TEAM == 8) {
     // two teams share a DPP row: lanes 0-7 take lane K, lanes 8-15 lane 8 + K. The first move fills
@@ -61,9 +94,17 @@ __device__ __forceinline__ double bcast(double x) {
 
 template <int TEAM>
 __device__ __forceinline__ double team_sum(double x) {
+  if constexpr (TEAM == 12) {
+    // sums of aligned groups of four lanes (team bases are multiples of 4), then the team's three groups
+    x += __shfl_xor(x, 1, 64);
+    x += __shfl_xor(x, 2, 64);
+    const int b = team_base_in_wave<12>();
+    return __shfl(x, b, 64) + __shfl(x, b + 4, 64) + __shfl(x, b + 8, 64);
+  } else {
 #pragma unroll
-  for (int m = 1; m < TEAM; m <<= 1) x += __shfl_xor(x, m, TEAM);
-  return x;
+    for (int m = 1; m < TEAM; m <<= 1) x += __shfl_xor(x, m, TEAM);
+    return x;
+  }
 }
 
 // ---- parameters --------------------------------------------------------------------------------

@@ -3,8 +3,8 @@
 // written for CDNA4.
 //
 // Work mapping. A team of lanes simulates one (voxel, offset row, tangent pass): 8 lanes (two teams per
-// DPP row) when the augmented state has at most 8 entries (one CEST pool), 16 (one DPP row) up to 16
-// entries, 32 beyond.
+// DPP row) when the augmented state has at most 8 entries (one CEST pool), 12 lanes (five teams per
+// wave, broadcasts by ds_bpermute) up to 12 entries (two pools), 16 (one DPP row) up to 16, 32 beyond.
 //  * The generator X = [[A, C], [0, 0]] dt (matrix.py:83-295, expm.py:98-106) is sparse and
 //    identical on every lane of a team, so each lane rebuilds it from the parameters and applies it
 //    with no communication.
@@ -55,8 +55,8 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   constexpr int NT = 1 + NC, N = 3 * NT + 1;
   using I = PrmIdx<NC>;
   constexpr int PTA = PT > 0 ? PT : 1;
-  const int lane = threadIdx.x % TEAM;
-  const int tib = threadIdx.x / TEAM;
+  const int lane = team_lane<TEAM>();
+  const int tib = team_index<TEAM>();
   bool dead = team >= total;
   const long tt = dead ? 0 : team;
   const int o = (int)(tt % a.O);
@@ -156,14 +156,14 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
 template <int NC, int PT, int TEAM, bool KEEP>
 __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArgs a) {
   constexpr int N = 3 * (1 + NC) + 1;
-  constexpr int TPB = 256 / TEAM;
+  constexpr int TPB = teams_per_block<TEAM>();
   constexpr int PTA = PT > 0 ? PT : 1;
   __shared__ double sh_prm[TPB][PrmIdx<NC>::size];
   __shared__ double sh_state[TPB][(1 + PTA) * N];  // the state, parked while a block is chained
   __shared__ double sh_start[TPB][(1 + PTA) * N];  // the start state and its tangent seeds
   __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
   extern __shared__ double dyn_keep[];  // TPB keep slots when the plan holds KEEP ops
-  const int tib = threadIdx.x / TEAM;
+  const int tib = team_index<TEAM>();
   double* keep = KEEP ? dyn_keep + (long)tib * keep_slot_doubles<NC, PT>() : nullptr;
   sim_team<NC, PT, TEAM, false, KEEP>(a, a.team0 + (long)blockIdx.x * TPB + tib, (long)a.V * a.O * a.n_pass, a.V, sh_prm,
                          sh_state, sh_start, sh_tk, sh_ti, keep);
@@ -174,14 +174,14 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArg
 template <int NC, int PT, int TEAM, bool KEEP>
 __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_list_kernel(SimArgs a) {
   constexpr int N = 3 * (1 + NC) + 1;
-  constexpr int TPB = 256 / TEAM;
+  constexpr int TPB = teams_per_block<TEAM>();
   constexpr int PTA = PT > 0 ? PT : 1;
   __shared__ double sh_prm[TPB][PrmIdx<NC>::size];
   __shared__ double sh_state[TPB][(1 + PTA) * N];
   __shared__ double sh_start[TPB][(1 + PTA) * N];
   __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
   extern __shared__ double dyn_keep[];
-  const int tib = threadIdx.x / TEAM;
+  const int tib = team_index<TEAM>();
   double* keep = KEEP ? dyn_keep + (long)tib * keep_slot_doubles<NC, PT>() : nullptr;
   const int nv = *a.n_vlist;
   const long total = (long)nv * a.O * a.n_pass;
@@ -194,8 +194,8 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_list_kernel(S
 template <int NC, int PT, bool KEEP>
 static int launch_nc_pt_k(const SimArgs& a, hipStream_t stream) {
   constexpr int NA = 3 * (1 + NC) + 2;
-  constexpr int TEAM = NA <= 8 ? 8 : (NA <= 16 ? 16 : 32);  // one lane per propagator column
-  constexpr int TPB = 256 / TEAM;
+  constexpr int TEAM = NA <= 8 ? 8 : (NA <= 12 ? 12 : (NA <= 16 ? 16 : 32));  // one lane per propagator column
+  constexpr int TPB = teams_per_block<TEAM>();
   const long teams = (long)a.V * a.O * a.n_pass;
   const size_t dyn = KEEP ? (size_t)TPB * keep_slot_doubles<NC, PT>() * sizeof(double) : 0;
   if (dyn > 65536) {  // gfx950 gives a workgroup up to 160 KiB of LDS; dynamic LDS above 64 KiB is opt-in
