@@ -29,9 +29,9 @@ sys.path[:0] = [os.path.join(ROOT, "bmc-jax_amd")]
 FREE = [("cest_k", 0), ("cest_f", 0), ("water_r1", None), ("water_r2", None), ("b0_inhom", None)]
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per voxel of one launch of this workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 2.531e11 B
-# per 65,536-voxel launch (40.5 GB fetched, 212.6 GB written), almost all register-spill traffic.
-TRAFFIC_BYTES_PER_VOXEL = 2.531e11 / 65536
+# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 2.459e11 B
+# per 65,536-voxel launch (45.6 GB fetched, 200.3 GB written), almost all register-spill traffic.
+TRAFFIC_BYTES_PER_VOXEL = 2.459e11 / 65536
 
 
 def workload(n_vox, seed=2):
