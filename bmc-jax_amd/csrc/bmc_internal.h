@@ -85,6 +85,7 @@ struct SimArgs {
   long team0;             // first team of this launch (a launch covers at most 2^31 threads)
   const int32_t* vlist;   // [<= V] or null: simulate only these voxels (compacted list)
   const int32_t* n_vlist; // device count of vlist
+  int t_off;              // first tangent of this launch's pass 0 (a launch may carry the last pass alone)
   const double* rot;      // [n_ops, 2] (cos d, sin d) of REUSE ops
   int keep;               // the plan holds KEEP ops: the launch gives every team an LDS slot for one propagator
 };

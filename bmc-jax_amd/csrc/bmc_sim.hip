@@ -87,7 +87,7 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   double* st0 = sh_start[tib];
   for (int i = lane; i < (1 + PTA) * N; i += TEAM) {
     const int p = i / N - 1, r = i % N;
-    const int gp = q * PT + p;
+    const int gp = a.t_off + q * PT + p;
     double val = 0.0;
     if (p < 0)
       val = load_real(a.m0, (long)v * N + r, a.dtype);
@@ -116,7 +116,7 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   int kind[PTA], kidx[PTA];
 #pragma unroll
   for (int p = 0; p < PTA; p++) {
-    const int gp = q * PT + p;
+    const int gp = a.t_off + q * PT + p;
     kind[p] = (p < PT && gp < a.P) ? sh_tk[gp] : TK_NONE;
     kidx[p] = (p < PT && gp < a.P) ? sh_ti[gp] : 0;
   }
@@ -133,7 +133,7 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
                         a.rot, keep);
   if (dead || lane != 0) return;
   const long vo = (long)v * a.O + o;
-  if (q == 0) {
+  if (q == 0 && a.t_off == 0) {
     if (a.m_out)
       for (int i = 0; i < N; i++) store_real(a.m_out, vo * N + i, M[i], a.dtype);
     if (a.z_out) a.z_out[vo] = M[a.jc];
@@ -141,7 +141,7 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   if (a.jac_out) {
 #pragma unroll
     for (int p = 0; p < PTA; p++) {
-      const int gp = q * PT + p;
+      const int gp = a.t_off + q * PT + p;
       if (p < PT && gp < a.P) {
         double val = 0.0;
 #pragma unroll
@@ -244,7 +244,7 @@ static int launch_nc(const SimArgs& a, hipStream_t stream) {
   }
 }
 
-int launch_sim(int nc, const SimArgs& a, hipStream_t stream) {
+static int launch_sim_nc(int nc, const SimArgs& a, hipStream_t stream) {
   switch (nc) {
     case 0:
       return launch_nc<0>(a, stream);
@@ -260,6 +260,23 @@ int launch_sim(int nc, const SimArgs& a, hipStream_t stream) {
       set_error("simulate: n_cest must be 0.." + std::to_string(BMC_MAX_CEST));
       return BMC_ENOTSUP;
   }
+}
+
+// P tangents in passes of PT: when the last pass would carry fewer than PT, it runs as its own launch
+// with that many tangents per pass, so no lane works on an empty tangent slot (5 tangents: 2 + 2 + 1).
+int launch_sim(int nc, const SimArgs& a, hipStream_t stream) {
+  const int P = a.jac_out ? a.P : 0;
+  if (a.PT <= 1 || a.n_pass <= 1 || a.t_off != 0 || P % a.PT == 0 || a.traj)
+    return launch_sim_nc(nc, a, stream);
+  SimArgs head = a;
+  head.n_pass = a.n_pass - 1;
+  int rc = launch_sim_nc(nc, head, stream);
+  if (rc != BMC_OK) return rc;
+  SimArgs tail = a;
+  tail.t_off = (a.n_pass - 1) * a.PT;
+  tail.PT = P - tail.t_off;
+  tail.n_pass = 1;
+  return launch_sim_nc(nc, tail, stream);
 }
 
 int tan_kind_of_slot(int slot, int nc, int* idx) {
