@@ -29,9 +29,10 @@ sys.path[:0] = [os.path.join(ROOT, "bmc-jax_amd")]
 FREE = [("cest_k", 0), ("cest_f", 0), ("water_r1", None), ("water_r2", None), ("b0_inhom", None)]
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per voxel of one launch of this workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 2.459e11 B
-# per 65,536-voxel launch (45.6 GB fetched, 200.3 GB written), almost all register-spill traffic.
-TRAFFIC_BYTES_PER_VOXEL = 2.459e11 / 65536
+# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 1.698e11 B
+# per 65,536-voxel step (its two dispatches: 31.7 GB fetched, 138.1 GB written), almost all
+# register-spill traffic.
+TRAFFIC_BYTES_PER_VOXEL = 1.698e11 / 65536
 
 
 def workload(n_vox, seed=2):
@@ -640,7 +641,8 @@ def main():
         "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (MI355X's FP64 vector "
                      "and FP64 matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES_PER_VOXEL * V,
-                     "traffic_unit": "bytes per launch (measured, scaled by voxels)",
+                     "traffic_unit": "bytes per step (measured, scaled by voxels)",
+                     "launches_per_step": "2 (tangent passes 2 + 2, then the fifth tangent alone)",
                      "flops_per_launch": flops_launch, "avg_launch_ms": 1e3 * avg_launch_s},
     }
     if not args.no_cpu:
