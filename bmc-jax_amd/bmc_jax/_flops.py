@@ -84,19 +84,20 @@ def _gen_l1(prm, nc, dt, amp, cph, sph, fr):
     return n1 * dt[None, :]
 
 
-def degree_and_squarings(l1):
+def degree_and_squarings(l1, mmax=M_MAX):
+    """The kernels' Taylor degree m and squaring count s for 1-norms l1, with degree cap mmax."""
     l1 = np.asarray(l1, np.float64)
-    m = np.searchsorted(THETA[1:M_MAX + 1], l1, side="left") + 1     # smallest m with theta_m >= l1
-    m = np.clip(m, 1, M_MAX)
-    s = np.where(l1 <= THETA[M_MAX], 0, np.ceil(np.log2(np.maximum(l1, 1e-300) / THETA[M_MAX])))
+    m = np.searchsorted(THETA[1:mmax + 1], l1, side="left") + 1     # smallest m with theta_m >= l1
+    m = np.clip(m, 1, mmax)
+    s = np.where(l1 <= THETA[mmax], 0, np.ceil(np.log2(np.maximum(l1, 1e-300) / THETA[mmax])))
     s = np.maximum(s, 0).astype(np.int64)
     big = s > 0
     while True:
-        bad = big & (l1 * np.ldexp(1.0, -s) > THETA[M_MAX])
+        bad = big & (l1 * np.ldexp(1.0, -s) > THETA[mmax])
         if not bad.any():
             break
         s = s + bad
-    m = np.where(big, M_MAX, m)
+    m = np.where(big, mmax, m)
     return m.astype(np.int64), s
 
 
@@ -230,9 +231,12 @@ def single_event_rows(events):
     return live
 
 
+CWJ_MMAX = 18   # the single-event Jacobian kernel's Taylor degree cap (csrc/bmc_cwj.h)
+
+
 def reverse_flops(prm, nc, events, slots, live, chunk=2):
     """Useful float64 flops per voxel of prm [V, S] of one single-event Jacobian launch (csrc/bmc_cwj.hip),
-    summed over rows, with the forward kernel's Taylor degree / squaring rule:
+    summed over rows, with the kernel's Taylor degree / squaring rule (degree cap CWJ_MMAX):
       the Taylor columns of T            NA m (FX + 2N)
       per squaring level (per lane)      E^2 and E^T G: 2 N^2 each; the rows of E^2 and G E^T: 2 N NA each
       the state                          2 N NA
@@ -251,7 +255,7 @@ def reverse_flops(prm, nc, events, slots, live, chunk=2):
         ph = ev["rf_phase"][o, e].astype(np.float64)
         l1 = _gen_l1(prm, nc, ev["dt"][o, e].astype(np.float64), ev["rf_amp"][o, e].astype(np.float64), np.cos(ph),
                      np.sin(ph), ev["rf_freq"][o, e].astype(np.float64))
-        m, s = degree_and_squarings(l1)
+        m, s = degree_and_squarings(l1, CWJ_MMAX)
         m, s = m[:, 0], s[:, 0]
         f = NA * m * (FX + 2 * N) + s * NA * (4 * N * N + 4 * N * NA) + 2 * N * NA
         for ch in chunks:

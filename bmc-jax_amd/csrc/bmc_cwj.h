@@ -7,6 +7,10 @@
 namespace bmc {
 
 constexpr int kCwjChunk = 2;  // tangents per Taylor pass
+// Taylor degree cap of this kernel. With the tangents contracted, a squaring level costs about 1.4
+// Taylor terms of the series (primal and every tangent pass), so a lower cap with more squarings is
+// cheaper than the forward kernel's 30; 18 keeps the state within 1e-12 of it (1.7 more levels).
+constexpr int kCwjMMax = 18;
 
 // One level: G <- G E^T + E^T G, then E <- E E (columns R, rows Er), lane j's share.
 template <int NC, int TEAM>
@@ -79,7 +83,7 @@ __device__ __forceinline__ void cwj_row(const double* prm, const double* m0, con
     int k0[1] = {TK_NONE}, i0[1] = {0};
     TanCo tc0[1];
     double l1, sc;
-    build_gen<NC, 0>(prm, e, mc, k0, i0, G, tc0, l1, sc, nan_l1, m, s, isn, kMMax);
+    build_gen<NC, 0>(prm, e, mc, k0, i0, G, tc0, l1, sc, nan_l1, m, s, isn, kCwjMMax);
     double dR0[1][N];
     taylor<NC, 0>(G, tc0, k0, i0, dwp, m, R, (lane == N) ? 1.0 : 0.0, nullptr, R, dR0);  // column lane of T
     // rows of T: Er[l] = T[lane][l]
@@ -117,7 +121,7 @@ __device__ __forceinline__ void cwj_row(const double* prm, const double* m0, con
       int mq, sq;
       bool isq;
       Gen<NC> Gq;
-      build_gen<NC, kCwjChunk>(prm, e, mc, kind, kidx, Gq, tc, l1, sc, nan_l1, mq, sq, isq, kMMax);
+      build_gen<NC, kCwjChunk>(prm, e, mc, kind, kidx, Gq, tc, l1, sc, nan_l1, mq, sq, isq, kCwjMMax);
       double T[N], dT[kCwjChunk][N];
 #pragma unroll
       for (int i = 0; i < N; i++) T[i] = (i == lane) ? 1.0 : 0.0;

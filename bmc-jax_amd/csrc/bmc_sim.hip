@@ -264,9 +264,13 @@ static int launch_sim_nc(int nc, const SimArgs& a, hipStream_t stream) {
 
 // P tangents in passes of PT: when the last pass would carry fewer than PT, it runs as its own launch
 // with that many tangents per pass, so no lane works on an empty tangent slot (5 tangents: 2 + 2 + 1).
+// Only launches whose one pass fills the GPU many times over are split: a small launch (a few tubes)
+// is latency-bound, and two launches in a row would take about twice as long as one.
+constexpr long kSplitMinTeams = 1L << 17;
+
 int launch_sim(int nc, const SimArgs& a, hipStream_t stream) {
   const int P = a.jac_out ? a.P : 0;
-  if (a.PT <= 1 || a.n_pass <= 1 || a.t_off != 0 || P % a.PT == 0 || a.traj)
+  if (a.PT <= 1 || a.n_pass <= 1 || a.t_off != 0 || P % a.PT == 0 || a.traj || (long)a.V * a.O < kSplitMinTeams)
     return launch_sim_nc(nc, a, stream);
   SimArgs head = a;
   head.n_pass = a.n_pass - 1;

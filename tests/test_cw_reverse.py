@@ -106,7 +106,9 @@ def test_reverse_jacobian_matches_forward_mode(nc):
             os.environ.pop("BMC_FORWARD_JACOBIAN", None)
         out[mode] = (m.cpu().numpy(), J.cpu().numpy())
     (mr, Jr), (mf, Jf) = out["reverse"], out["forward"]
-    assert np.max(np.abs(mr - mf)) < 1e-12
+    # the kernels split exp(X) differently (Taylor degree cap 18 with more squarings against 30):
+    # the states agree to rounding, within the 1e-11 that both keep against the float64 oracle
+    assert np.max(np.abs(mr - mf)) < 1e-11
     for p in range(len(slots)):
         scale = max(np.abs(Jf[..., p]).max(), 1e-30)
         assert np.max(np.abs(Jr[..., p] - Jf[..., p])) / scale < 1e-9, names[p]

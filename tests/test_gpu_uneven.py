@@ -143,7 +143,8 @@ def test_reverse_jacobian_sweep_matches_forward_mode(problems, monkeypatch):
     z_r = z_r.cpu().numpy()
     monkeypatch.setenv("BMC_FORWARD_JACOBIAN", "1")
     z_f, J_f = b.simulate(x)
-    np.testing.assert_allclose(z_r, z_f.cpu().numpy(), rtol=1e-12, atol=1e-14)
+    # different Taylor degree / squaring splits of exp(X): equal to rounding, the oracle's 1e-11 bar
+    np.testing.assert_allclose(z_r, z_f.cpu().numpy(), rtol=0, atol=1e-11)
     for i in range(len(problems)):
         jr, jf = b.jacobian_of(J_r, i), b.jacobian_of(J_f, i)
         assert np.max(np.abs(jr - jf)) <= 1e-9 * np.abs(jf).max(), i
