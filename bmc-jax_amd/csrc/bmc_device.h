@@ -87,23 +87,45 @@ __device__ __forceinline__ double bcast(double x) {
     // the row with lane K, the second overwrites lanes 8-15 only (bank_mask 0xc), so no select is needed.
     const double all = __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);
     return __builtin_amdgcn_update_dpp(all, x, 0x150 + 8 + K, 0xf, 0xc, false);
+  } else if constexpr (TEAM == 32 && K < 16) {
+    // a 32-lane team spans two DPP rows: lane K to every lane of its row (row_newbcast), then lane 15
+    // of the team's first row, which now holds it, to the second row (row_bcast:15, rows 1 and 3 only)
+    const double t = __builtin_amdgcn_mov_dpp(x, 0x150 + K, 0xf, 0xf, true);
+    return __builtin_amdgcn_update_dpp(t, t, 0x142, 0xa, 0xf, false);
   } else {
     return __shfl(x, K, TEAM);
   }
 }
 
+// x plus the value of another lane of its DPP row, picked by the DPP control `CTRL`.
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double x) {
+  return x + __builtin_amdgcn_mov_dpp(x, CTRL, 0xf, 0xf, true);
+}
+
+// Sum over the team, on every lane of it. Teams of up to 16 lanes reduce with DPP moves inside their row:
+// lane ^ 1 (quad_perm [1,0,3,2]), lane ^ 2 (quad_perm [2,3,0,1]), then the mirrored quad of the half row
+// and the mirrored half of the row. 32-lane teams keep ds_bpermute butterflies, which timed faster for
+// the four-pool forward kernel.
 template <int TEAM>
 __device__ __forceinline__ double team_sum(double x) {
-  if constexpr (TEAM == 12) {
-    // sums of aligned groups of four lanes (team bases are multiples of 4), then the team's three groups
-    x += __shfl_xor(x, 1, 64);
-    x += __shfl_xor(x, 2, 64);
-    const int b = team_base_in_wave<12>();
-    return __shfl(x, b, 64) + __shfl(x, b + 4, 64) + __shfl(x, b + 8, 64);
-  } else {
+  static_assert(TEAM == 8 || TEAM == 12 || TEAM == 16 || TEAM == 32, "team size");
+  if constexpr (TEAM == 32) {
 #pragma unroll
     for (int m = 1; m < TEAM; m <<= 1) x += __shfl_xor(x, m, TEAM);
     return x;
+  } else {
+    x = dpp_add<0xb1>(x);
+    x = dpp_add<0x4e>(x);
+    if constexpr (TEAM == 12) {
+      // the team's three quads (team bases are multiples of 4)
+      const int b = team_base_in_wave<12>();
+      return __shfl(x, b, 64) + __shfl(x, b + 4, 64) + __shfl(x, b + 8, 64);
+    } else {
+      x = dpp_add<0x141>(x);                            // row_half_mirror
+      if constexpr (TEAM == 16) x = dpp_add<0x140>(x);  // row_mirror
+      return x;
+    }
   }
 }
 
