@@ -135,7 +135,8 @@ __global__ void __launch_bounds__(256, BMC_CWJ_WAVES_PER_SIMD) cwj_list_kernel(S
 template <int NC>
 static int launch_cwj_nc(const SimArgs& a, const int32_t* row_live, hipStream_t stream) {
   constexpr int NA = 3 * (1 + NC) + 2;
-  constexpr int TEAM = NA <= 8 ? 8 : (NA <= 16 ? 16 : 32);
+  // four CEST pools (NA = 17) run on 16-lane teams with the augmented column spread (cwj_row)
+  constexpr int TEAM = NA <= 8 ? 8 : (NA <= 17 ? 16 : 32);
   constexpr int TPB = 256 / TEAM;
   const long teams = (long)a.V * a.O;
   if (a.vlist) {

@@ -5,7 +5,8 @@
 //
 // The simulation is one persistent launch per sweep. Its work items are (problem, row, tangent pass)
 // triples, grouped into chunks of same-class items (one chunk per wave: 8 teams of 8 lanes for one
-// CEST pool, 4 of 16 for two or three, 2 of 32 for four). A fixed grid of one-wave workgroups pulls chunks from a global
+// CEST pool, 4 of 16 for two or three, 2 of 32 for four; the reverse Jacobian kernel runs four pools
+// as 4 teams of 16). A fixed grid of one-wave workgroups pulls chunks from a global
 // counter until the list is exhausted, so long and short problems share the machine without a
 // launch per shape. Each team runs the same row program as bmc_simulate (run_row).
 // The LM steps (float64 normal equations, damped solve, clip, accept/reject) are ragged versions of
@@ -261,7 +262,7 @@ __global__ void __launch_bounds__(64, BMC_USIM_WAVES_PER_SIMD) ucwj_kernel(USimA
         u_chunk_cwj<3, 16>(a, ch, sh_prm, sh_start);
         break;
       default:
-        u_chunk_cwj<4, 32>(a, ch, sh_prm, sh_start);
+        u_chunk_cwj<4, 16>(a, ch, sh_prm, sh_start);  // column N spread over the team (cwj_row)
         break;
     }
     __syncthreads();
@@ -456,7 +457,8 @@ struct bmc_ubatch {
   int PT = 1, grid = 1024, order = 1;
   UItem *d_items_j = nullptr, *d_items_p = nullptr;
   int2 *d_chunks_j = nullptr, *d_chunks_p = nullptr;
-  int n_chunks_j = 0, n_chunks_p = 0;
+  int2* d_chunks_r = nullptr;  // the primal items, chunked for the reverse Jacobian kernel's teams
+  int n_chunks_j = 0, n_chunks_p = 0, n_chunks_r = 0;
   std::vector<void*> cfg_ptrs;
   int* d_counter = nullptr;
   ~bmc_ubatch() {
@@ -468,8 +470,9 @@ namespace {
 
 // Items of every (problem, row, pass) with `passes(P)` passes, grouped into same-class chunks of
 // 64 / TEAM items. order 0: problem order; 1: classes with more pools first, then more rows first.
+// per4: teams per wave of four-pool items (2 for 32-lane teams, 4 for the reverse kernel's 16-lane ones).
 void build_items(const bmc_ubatch& b, int PT, int order, bool tangents, std::vector<UItem>& items,
-                 std::vector<int2>& chunks) {
+                 std::vector<int2>& chunks, int per4 = 2) {
   std::vector<int> ids(b.n_prob);
   std::iota(ids.begin(), ids.end(), 0);
   if (order == 1)
@@ -481,7 +484,7 @@ void build_items(const bmc_ubatch& b, int PT, int order, bool tangents, std::vec
   items.clear();
   chunks.clear();
   for (int cls = 3; cls >= 0; cls--) {
-    const int per = (cls == 3) ? 2 : (cls == 0 ? 8 : 4);  // teams per wave: 32, 16 or 8 lanes
+    const int per = (cls == 3) ? per4 : (cls == 0 ? 8 : 4);  // teams per wave: 32, 16 or 8 lanes
     std::vector<UItem> mine;
     for (int id : ids) {
       const UProb& p = b.h_probs[id];
@@ -515,6 +518,18 @@ int upload(bmc_ubatch* b, const std::vector<UItem>& items, const std::vector<int
   return BMC_OK;
 }
 
+int upload_chunks(bmc_ubatch* b, const std::vector<int2>& chunks, int2** dc) {
+  void* pc = nullptr;
+  if (hipMalloc(&pc, std::max<size_t>(chunks.size(), 1) * sizeof(int2)) != hipSuccess) {
+    set_error("uneven batch: device allocation failed");
+    return BMC_ENOMEM;
+  }
+  b->cfg_ptrs.push_back(pc);
+  BMC_HIP_CHECK(hipMemcpy(pc, chunks.data(), chunks.size() * sizeof(int2), hipMemcpyHostToDevice));
+  *dc = static_cast<int2*>(pc);
+  return BMC_OK;
+}
+
 USimArgs sim_args(const bmc_ubatch* b, bool tangents, const double* xw, double* z_out, double* jac_out,
                   const int32_t* active) {
   USimArgs a;
@@ -540,8 +555,8 @@ USimArgs sim_args(const bmc_ubatch* b, bool tangents, const double* xw, double* 
   a.t_scale = b->d_scale;
   a.xw = xw;
   a.items = tangents && !reverse ? b->d_items_j : b->d_items_p;
-  a.chunks = tangents && !reverse ? b->d_chunks_j : b->d_chunks_p;
-  a.n_chunks = tangents && !reverse ? b->n_chunks_j : b->n_chunks_p;
+  a.chunks = reverse ? b->d_chunks_r : (tangents ? b->d_chunks_j : b->d_chunks_p);
+  a.n_chunks = reverse ? b->n_chunks_r : (tangents ? b->n_chunks_j : b->n_chunks_p);
   a.row_live = reverse ? p.d_live : nullptr;
   a.counter = b->d_counter;
   a.z_out = z_out;
@@ -683,6 +698,10 @@ extern "C" int bmc_ubatch_configure(bmc_ubatch* b, int tangents_per_pass, int gr
   rc = upload(b, items, chunks, &b->d_items_p, &b->d_chunks_p);
   if (rc != BMC_OK) return rc;
   b->n_chunks_p = (int)chunks.size();
+  build_items(*b, b->PT, order, false, items, chunks, 4);  // the same items, four 16-lane teams per wave
+  rc = upload_chunks(b, chunks, &b->d_chunks_r);
+  if (rc != BMC_OK) return rc;
+  b->n_chunks_r = (int)chunks.size();
   return BMC_OK;
 }
 
