@@ -234,14 +234,16 @@ def single_event_rows(events):
 CWJ_MMAX = 18   # the single-event Jacobian kernel's Taylor degree cap (csrc/bmc_cwj.h)
 
 
-def reverse_flops(prm, nc, events, slots, live, chunk=2):
+def reverse_flops(prm, nc, events, slots, live, chunk=None):
     """Useful float64 flops per voxel of prm [V, S] of one single-event Jacobian launch (csrc/bmc_cwj.hip),
     summed over rows, with the kernel's Taylor degree / squaring rule (degree cap CWJ_MMAX):
       the Taylor columns of T            NA m (FX + 2N)
       per squaring level (per lane)      E^2 and E^T G: 2 N^2 each; the rows of E^2 and G E^T: 2 N NA each
       the state                          2 N NA
       per chunk of T <= `chunk` tangents NA m (FX (1 + T) + dX + 2N (1 + T)), then 2 N NA + NA per tangent
-    live: single_event_rows(events)."""
+    live: single_event_rows(events). chunk defaults to the kernel's: 2, or 1 at four CEST pools."""
+    if chunk is None:
+        chunk = 1 if nc >= 4 else 2
     ev = {k: np.asarray(getattr(events, k) if not isinstance(events, dict) else events[k])
           for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
     N = 3 * (1 + nc) + 1

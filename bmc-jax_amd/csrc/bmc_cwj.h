@@ -6,7 +6,12 @@
 
 namespace bmc {
 
-constexpr int kCwjChunk = 2;  // tangents per Taylor pass
+// Tangents per Taylor pass: two share one primal series; at four CEST pools one, since two spill in the
+// Horner loop itself (one tangent per pass halved the four-pool sweep).
+template <int NC>
+__host__ __device__ constexpr int cwj_chunk() {
+  return NC >= 4 ? 1 : 2;
+}
 // Taylor degree cap of this kernel. With the tangents contracted, a squaring level costs about 1.4
 // Taylor terms of the series (primal and every tangent pass), so a lower cap with more squarings is
 // cheaper than the forward kernel's 30; 18 keeps the state within 1e-12 of it (1.7 more levels).
@@ -182,6 +187,7 @@ __device__ __forceinline__ void cwj_row(const double* prm, const double* m0, con
   if (lane == 0) store_state(M, isn);
   if (!want_jac) return;
   // tangents: dZ/dp = <dT_p, G>, dT_p by forward-mode Horner, kCwjChunk tangents per pass
+  constexpr int kCwjChunk = cwj_chunk<NC>();
   for (int q0 = 0; q0 < P; q0 += kCwjChunk) {
     int kind[kCwjChunk], kidx[kCwjChunk];
 #pragma unroll

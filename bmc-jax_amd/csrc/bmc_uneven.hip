@@ -238,8 +238,12 @@ __global__ void __launch_bounds__(64, BMC_USIM_WAVES_PER_SIMD) usim_kernel(USimA
   }
 }
 
+#ifndef BMC_UCWJ_WAVES_PER_SIMD
+#define BMC_UCWJ_WAVES_PER_SIMD BMC_USIM_WAVES_PER_SIMD
+#endif
+
 // The Jacobian sweep of single-event rows: persistent over the one-item-per-row chunks.
-__global__ void __launch_bounds__(64, BMC_USIM_WAVES_PER_SIMD) ucwj_kernel(USimArgs a) {
+__global__ void __launch_bounds__(64, BMC_UCWJ_WAVES_PER_SIMD) ucwj_kernel(USimArgs a) {
   __shared__ double sh_prm[8][PrmIdx<4>::size];
   __shared__ double sh_start[8][(1 + 3) * kUMaxN];
   __shared__ int sh_chunk;
