@@ -9,7 +9,7 @@ count s (the same 1-norm and theta_m table) and counts the useful float64 operat
                             an event that needed squarings; other chained events need none)
   apply to the state        (N + 1)(N + 3 PT N) + N (1 + PT) N adds of the team reduction
 Only the N + 1 columns of a propagator count, not the idle lanes of a 16-lane team.
-`reference_flops` is the survey's model of the reference's per-event work (every event its own
+`reference_flops` is a model of the reference's per-event work (every event its own
 Pade exponential with its tangents, reference float32 degree/squaring rule).
 """
 import math
@@ -154,7 +154,7 @@ def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1):
 
 
 def reference_flops(n_aug, P, m_ref, s_ref):
-    """The survey's model of the reference's per-event work: expm + P tangents + apply (flops)."""
+    """A model of the reference's per-event work: expm + P tangents + apply (flops)."""
     km = {3: 2, 5: 3, 7: 4, 9: 5, 13: 6}[m_ref]
     n3, n2 = n_aug ** 3, n_aug ** 2
     return 2 * n3 * (km + s_ref) + 8.0 / 3.0 * n3 + P * (4 * n3 * (km + s_ref) + 4 * n3) + 2 * n2 + 4 * P * n2
