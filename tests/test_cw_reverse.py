@@ -138,18 +138,21 @@ def test_single_event_rule():
 
 
 @pytest.mark.gpu
-def test_reverse_kernel_edge_rows():
+@pytest.mark.parametrize("nc", [1, 4])
+def test_reverse_kernel_edge_rows(nc):
     """Rows with no propagating event (Z = M0[jc], zero Jacobian), a row past the reference's NaN rule
-    (squaring count above 16: NaN state and Jacobian) and ordinary CW rows, reverse against forward."""
+    (squaring count above 16: NaN state and Jacobian) and ordinary CW rows, reverse against forward;
+    at four CEST pools the kernel spreads the augmented column over its 16-lane team."""
     import torch
     from bmc_jax import _native as nat
     nat.torch_cuda()
-    p = O.make_params(1.3, 0.075, pools=[dict(f=9.01e-4, k=50.0, dw=3.5, t1=1.3, t2=0.1)], b0=3.0)
+    pools = [dict(f=9.01e-4 * (i + 1), k=50.0 * (2 ** i), dw=3.5 - 1.1 * i, t1=1.3, t2=0.1) for i in range(nc)]
+    p = O.make_params(1.3, 0.075, pools=pools, b0=3.0, n_cest=nc)
     ev = O.cw_saturation(np.linspace(-6, 6, 6), b1_uT=1.0, tsat_s=2.0, b0_T=3.0)
     ev = {k: np.array(v, copy=True) for k, v in ev.items()}
     ev["dt"][1, 0] = 0.0            # no propagating event
     ev["dt"][2, 0] = 5e3            # 1-norm far beyond theta_ref 2^17: the reference gives NaN
-    slots = [OC.slot_index("cest_k", 0, 1), OC.slot_index("cest_f", 0, 1), OC.slot_index("water_r2", None, 1)]
+    slots = [OC.slot_index("cest_k", 0, nc), OC.slot_index("cest_f", 0, nc), OC.slot_index("water_r2", None, nc)]
     plan = nat.Plan(ev["type"], ev["dt"], ev["rf_amp"], ev["rf_phase"], ev["rf_freq"])
     recs = [p, p]
     prm = nat.device_array(pack_many(recs), torch.float64)
@@ -160,7 +163,7 @@ def test_reverse_kernel_edge_rows():
         if mode == "forward":
             os.environ["BMC_FORWARD_JACOBIAN"] = "1"
         try:
-            m, J = nat.simulate(plan, prm, 1, nat.BMC_F64, m0, slots=slots, scales=[1.0] * 3, jc=4)
+            m, J = nat.simulate(plan, prm, nc, nat.BMC_F64, m0, slots=slots, scales=[1.0] * 3, jc=2 * (1 + nc))
             torch.cuda.synchronize()
         finally:
             os.environ.pop("BMC_FORWARD_JACOBIAN", None)
@@ -172,5 +175,5 @@ def test_reverse_kernel_edge_rows():
     np.testing.assert_array_equal(mr[:, 1], m0s)           # nothing propagates: the start state
     assert np.all(Jr[:, 1] == 0.0)
     ok = [0, 3, 4, 5]
-    assert np.max(np.abs(mr[:, ok] - mf[:, ok])) < 1e-12
+    assert np.max(np.abs(mr[:, ok] - mf[:, ok])) < 1e-11
     assert np.max(np.abs(Jr[:, ok] - Jf[:, ok])) < 1e-9 * np.abs(Jf[:, ok]).max()
