@@ -118,6 +118,10 @@ int bmc_plan_program(const bmc_plan* plan, int64_t capacity, int32_t* ops_out, i
  *   jac_out_dev  [V, O, P] or NULL           d state[jac_component] / d x_p
  * NaN semantics follow the reference: an event whose reference squaring count exceeds 16
  * (simulation/expm.py:83) makes the state NaN until the next ADC reset.
+ * When every row of the plan propagates at most one event (CW saturation rows: no spoiler or gradient,
+ * a readout only as a row's last event) and dm0_dev is NULL, the tangents come from the reverse-
+ * accumulation kernel instead (same outputs, to rounding); BMC_FORWARD_JACOBIAN=1 in the environment
+ * keeps forward mode. bmc_lm_fit takes the same route for its Jacobian sweeps.
  */
 int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
                  const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
