@@ -171,7 +171,7 @@ def bench_design(args):
                                   "x 40 + gaps + 1 s recovery (10,283 events), 31 offsets, "
                                   f"{truths.shape[0]} parameter sets, d = 256 amplitudes + 31 offsets",
                       "truths": int(truths.shape[0]), "offsets": 31, "events_per_offset": int(ev_np["dt"].shape[1])},
-           "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (adjoint kernel)",
+           "roofline": {"bound": "valu", "bound_detail": "compute-bound in float64 vector FMA (adjoint kernel)",
                         "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "flops_per_launch": flops,
                         "avg_launch_ms": adj_ms},
@@ -275,7 +275,7 @@ def bench_uneven(args):
     ops, rp = b.plan.program()
     ev_all = {k: np.asarray(getattr(b.plan_events, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
     # single-event rows take the reverse-accumulation sweep (csrc/bmc_cwj.hip): count its own flops
-    live = None if os.environ.get("BMC_FORWARD_JACOBIAN", "0") not in ("", "0") else _flops.single_event_rows(ev_all)
+    live = _flops.single_event_rows(ev_all)
     rng = np.random.default_rng(0)
     sample = rng.choice(b.n, size=min(64, b.n), replace=False)
     fl = 0.0
@@ -314,7 +314,7 @@ def bench_uneven(args):
                                   "1-4 CEST pools + optional MT, 2-8 free values, 30 LM iterations in one hipGraph",
                       "problems": b.n, "rows": int(b.R), "values": int(b.n_values),
                       "autotuned": {"tangents_per_pass": best[0], "grid": best[1], "order": best[2]}},
-           "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (persistent "
+           "roofline": {"bound": "valu", "bound_detail": "compute-bound in float64 vector FMA (persistent "
                         + ("reverse-accumulation Jacobian sweep)" if live is not None else "sweep)"),
                         "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "flops_per_launch": flops_sweep,
@@ -638,7 +638,7 @@ def main():
                                f"{V} voxels, 5 free values, float32 I/O",
                    "voxels": V, "offsets": 61, "events_per_offset": int(ev_np['dt'].shape[1]), "free_values": 5,
                    "io_dtype": "float32", "plan_built_events_per_offset": stats["n_built"] / 61},
-        "roofline": {"bound": "mfma", "bound_detail": "compute-bound in float64 vector FMA (MI355X's FP64 vector "
+        "roofline": {"bound": "valu", "bound_detail": "compute-bound in float64 vector FMA (MI355X's FP64 vector "
                      "and FP64 matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES_PER_VOXEL * V,
                      "traffic_unit": "bytes per step (measured, scaled by voxels)",

@@ -100,10 +100,12 @@ def simulate_trajectory(params: SimParams, events, M_init, n_voxels=None, dtype=
 
 
 def simulate_with_jac(params: SimParams, events, M_init, slots, scales, jc, dM_init=None, n_voxels=None, dtype=None,
-                      reset_after_adc=True, want_m=True):
+                      reset_after_adc=True, want_m=True, forward_jacobian=False):
     """Final states and d state[jc] / d x_p for packed slots (tangent scale applied).
 
-    dM_init: [V, P, N] derivative of M_init per tangent, or None. Returns (m [V,O,N] or None, J [V,O,P]).
+    dM_init: [V, P, N] derivative of M_init per tangent, or None. forward_jacobian: forward-mode tangents
+    even for single-event rows (the reverse kernel's result to rounding). Returns (m [V,O,N] or None,
+    J [V,O,P]).
     """
     prm, m0, nc, V, code, npd = prepare(params, M_init, n_voxels, dtype)
     plan = plan_for(events, reset_after_adc)
@@ -112,7 +114,8 @@ def simulate_with_jac(params: SimParams, events, M_init, slots, scales, jc, dM_i
         N = 3 * (1 + nc) + 1
         dm0 = _native.device_array(np.broadcast_to(np.asarray(dM_init), (V, len(slots), N)).astype(npd),
                                    _torch_dtype(npd))
-    m, J = _native.simulate(plan, prm, nc, code, m0, dm0=dm0, slots=slots, scales=scales, jc=jc, want_m=want_m)
+    m, J = _native.simulate(plan, prm, nc, code, m0, dm0=dm0, slots=slots, scales=scales, jc=jc, want_m=want_m,
+                            forward_jacobian=forward_jacobian)
     m = None if m is None else m.cpu().numpy().astype(npd, copy=False)
     J = np.zeros((V, plan.n_offsets, 0), npd) if J is None else J.cpu().numpy().astype(npd, copy=False)
     return m, J

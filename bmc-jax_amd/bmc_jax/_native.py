@@ -15,7 +15,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libbmc_hip.so")
 
 BMC_F32, BMC_F64 = 0, 1
-LM_EARLY_EXIT, LM_GRAPH, LM_START_CHECK = 1, 2, 4
+LM_EARLY_EXIT, LM_GRAPH, LM_START_CHECK, LM_FORWARD_JACOBIAN = 1, 2, 4, 8
+SIM_FORWARD_JACOBIAN = 1
+ABI_VERSION = 2
 MAX_CEST = 4
 MAX_TANGENTS = 16
 
@@ -71,6 +73,8 @@ def lib():
                                        ip32, dp, ip32, i64p]
         L.bmc_simulate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32, dp,
                                    ctypes.c_int, vp, vp, vp]
+        L.bmc_simulate_ex.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32,
+                                      dp, ctypes.c_int, vp, vp, ctypes.c_uint, vp]
         L.bmc_lm_fit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ip32, dp, dp, dp,
                                  ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_uint,
                                  ctypes.c_double, ctypes.c_double, vp, vp, vp, vp]
@@ -81,11 +85,12 @@ def lib():
                                            dp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int64, vp]
         for name in ("bmc_plan_create", "bmc_plan_create_ex", "bmc_plan_destroy", "bmc_plan_stats", "bmc_plan_program",
                      "bmc_plan_compile",
-                     "bmc_simulate", "bmc_simulate_trajectory", "bmc_lm_fit", "bmc_adjoint_workspace_size",
+                     "bmc_simulate", "bmc_simulate_ex", "bmc_simulate_trajectory", "bmc_lm_fit", "bmc_adjoint_workspace_size",
                      "bmc_simulate_adjoint"):
             getattr(L, name).restype = ctypes.c_int
-        if L.bmc_abi_version() != 1:
-            raise NativeError("libbmc_hip.so ABI version mismatch")
+        if L.bmc_abi_version() != ABI_VERSION:
+            raise NativeError(f"libbmc_hip.so has ABI version {L.bmc_abi_version()}, this binding needs {ABI_VERSION}: "
+                              "rebuild with python bmc-jax_amd/build.py")
         _lib = L
     return _lib
 
@@ -208,8 +213,9 @@ def device_array(x, dtype):
 
 
 def simulate(plan: Plan, params, n_cest: int, dtype_code: int, m0, dm0=None, slots=(), scales=(), jc=0,
-             want_m=True, want_jac=True):
-    """bmc_simulate on device tensors. params [V, S], m0 [V, N] torch tensors of the working dtype.
+             want_m=True, want_jac=True, forward_jacobian=False):
+    """bmc_simulate_ex on device tensors. params [V, S], m0 [V, N] torch tensors of the working dtype.
+    forward_jacobian: forward-mode tangents even where the reverse-accumulation kernel applies.
 
     Returns (m [V, O, N] or None, jac [V, O, P] or None) as device tensors.
     """
@@ -224,8 +230,9 @@ def simulate(plan: Plan, params, n_cest: int, dtype_code: int, m0, dm0=None, slo
     sl = (ctypes.c_int32 * max(P, 1))(*[int(s) for s in slots])
     sc = (ctypes.c_double * max(P, 1))(*[float(s) for s in scales])
     ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
-    check(L.bmc_simulate(plan.handle, V, n_cest, dtype_code, ptr(params), ptr(m0), ptr(dm0), P, sl, sc, jc,
-                         ptr(m_out), ptr(jac), stream_ptr()), "bmc_simulate")
+    flags = SIM_FORWARD_JACOBIAN if forward_jacobian else 0
+    check(L.bmc_simulate_ex(plan.handle, V, n_cest, dtype_code, ptr(params), ptr(m0), ptr(dm0), P, sl, sc, jc,
+                            ptr(m_out), ptr(jac), flags, stream_ptr()), "bmc_simulate_ex")
     return m_out, jac
 
 

@@ -1,13 +1,25 @@
 """Fits in which samples share values, or a value is free in some samples only
-(src/bmc_jax/fitting/joint.py:1-453).
+(src/bmc_jax/fitting/joint.py:165-453).
 
 The model and its Jacobian come from the GPU kernel: one launch per block of offsets simulates every
 sample (every isochromat of a B0 spread as an extra voxel) with one forward tangent per local entry.
 The tangent of a local entry is the kernel's derivative with respect to the slot's packed field,
 scaled per sample on the host for "factor" and spread nodes, with the start-state derivative passed
-as a seed where m_start is free. The Levenberg-Marquardt steps themselves are float64 on the host:
-one joint step for a coupled fit (joint.py:375-416), per-sample damping otherwise (joint.py:359-374).
+as a seed where m_start is free.
+
+The optimiser follows the reference's split (joint.py:357-416):
+
+* Separable fits (no shared and no per-block value) give every sample its own damping, `fit`'s LM.
+  When every slot is a plain value of a SimParams field and there is no B0 spread, the whole fit is
+  one `bmc_lm_fit` call with the free mask (joint.py:359-374 vmaps levenberg_marquardt the same way).
+  Factors, offsets, the start state and spreads have no kernel-side form of their write, so those
+  separable fits step on the host in float64 around GPU model evaluations.
+* Coupled fits take one damped step over the joint vector (joint.py:375-416): GPU model and
+  Jacobian, float64 step on the host.
 """
+from dataclasses import dataclass
+from typing import List, Tuple
+
 import numpy as np
 
 from .. import _engine, _native
@@ -22,68 +34,100 @@ PSEUDO = (M_START, B0_SPREAD)
 _FRACTION_FIELDS = {"water_f", "cest_f", "mt_f"}
 
 
-def _layout(spec, n, n_blocks):
-    """Where every free value lives (joint.py:70-96): free (n, L), pos (L,), n_loc, G (n, n_loc, P), cols."""
-    width = [n_blocks if s.scope == "block" else 1 for s in spec.slots]
-    pos = np.concatenate([[0], np.cumsum(width)[:-1]]).astype(int)
+@dataclass
+class _Unknowns:
+    """The joint vector of a fit and how it feeds every sample's local vector (joint.py:70-96).
+
+    A sample's local vector has one entry per slot, or one per block for a per-block slot, starting at
+    `first[l]`. Joint column p is `columns[p] = (slot, samples it is free in, block or -1)`; `feed[i, j, p]`
+    is 1 where column p sets local entry j of sample i."""
+    free: np.ndarray
+    first: np.ndarray
+    n_local: int
+    columns: List[Tuple[int, List[int], int]]
+    feed: np.ndarray
+
+    @property
+    def n_joint(self):
+        return len(self.columns)
+
+    def local(self, x):
+        """Joint vector (P,) -> local vectors (n, n_local)."""
+        return np.einsum("ijp,p->ij", self.feed, np.asarray(x, np.float64))
+
+    def joint_jacobian(self, Jl):
+        """Local Jacobians (n, points, n_local) -> the joint one (n * points, P)."""
+        return np.einsum("inj,ijp->inp", Jl, self.feed).reshape(-1, self.n_joint)
+
+
+def _unknowns(spec, n, n_blocks) -> _Unknowns:
+    per_slot = np.array([n_blocks if s.scope == "block" else 1 for s in spec.slots], int)
+    first = np.cumsum(per_slot) - per_slot
     free = np.zeros((n, spec.n), bool)
-    cols = []
+    columns = []
     for l, s in enumerate(spec.slots):
-        who = list(range(n)) if s.samples is None else list(s.samples)
-        missing = [i for i in who if not 0 <= i < n]
-        if missing:
-            raise ValueError(f"{s.name}: samples {missing} do not exist (the fit has {n})")
-        free[who, l] = True
-        if s.scope == "shared":
-            cols.append((l, who, -1))
-        elif s.scope == "block":
-            cols += [(l, who, b) for b in range(n_blocks)]
+        members = list(range(n)) if s.samples is None else [int(i) for i in s.samples]
+        outside = [i for i in members if i < 0 or i >= n]
+        if outside:
+            raise ValueError(f"{s.name}: samples {outside} do not exist in a fit of {n} samples")
+        free[members, l] = True
+        if s.scope == "sample":
+            columns.extend((l, [i], -1) for i in members)
+        elif s.scope == "shared":
+            columns.append((l, members, -1))
         else:
-            cols += [(l, [i], -1) for i in who]
-    G = np.zeros((n, int(sum(width)), len(cols)))
-    for p, (l, who, b) in enumerate(cols):
-        G[who, pos[l] + max(b, 0), p] = 1.0
-    return free, pos, int(sum(width)), G, cols
+            columns.extend((l, members, b) for b in range(n_blocks))
+    feed = np.zeros((n, int(per_slot.sum()), len(columns)))
+    for p, (l, members, b) in enumerate(columns):
+        feed[members, first[l] + max(b, 0), p] = 1.0
+    return _Unknowns(free, first, int(per_slot.sum()), columns, feed)
 
 
-def _carried(params, m0, spec, sig_held=None):
-    """(n, L): each slot's number where it is not free (joint.py:99-117)."""
-    out = []
+def _held_values(params, m0, spec, held_sigma=None):
+    """(n, L): the number each slot has where it is not fitted (joint.py:99-117): the field's own value,
+    the neutral factor 1, the neutral offset 0, for m_start the start state's water Mz relative to the
+    water fraction, for b0_spread the held spread width."""
     n = m0.shape[0]
-    for s in spec.slots:
+    table = np.empty((n, spec.n))
+    water_mz = 2 * (1 + n_cest_of(params))
+    for l, s in enumerate(spec.slots):
         if s.field == B0_SPREAD:
-            out.append(np.asarray(sig_held, np.float64))
-        elif s.mode == "factor":
-            out.append(np.ones(n))
-        elif s.mode == "add":
-            out.append(np.zeros(n))
+            table[:, l] = np.asarray(held_sigma, np.float64)
+        elif s.mode in ("factor", "add"):
+            table[:, l] = 1.0 if s.mode == "factor" else 0.0
         elif s.field == M_START:
-            nc = n_cest_of(params)
-            out.append(np.asarray(m0[:, 2 * (1 + nc)], np.float64) / np.asarray(params.water_f, np.float64))
+            table[:, l] = np.asarray(m0[:, water_mz], np.float64) / np.asarray(params.water_f, np.float64)
         else:
             a = np.asarray(getattr(params, s.field), np.float64)
-            out.append(a if s.index is None else a[:, s.index])
-    return np.stack(out, axis=1)
+            table[:, l] = a if s.index is None else a[:, s.index]
+    return table
 
 
-def _cov_to_sd_corr(cov, scale):
-    sd = np.sqrt(np.clip(np.diag(cov), 0.0, None))
+def _errors(cov, scale):
+    """Covariance in scaled units -> (standard deviations in physical units, correlation matrix)."""
+    sd = np.sqrt(np.maximum(np.diag(cov), 0.0))
     with np.errstate(invalid="ignore", divide="ignore"):
-        corr = cov / np.outer(sd, sd)
+        corr = cov / sd[:, None] / sd[None, :]
     return sd * scale, corr
 
 
 def _lm_trial(x, lam, J, r, lo, hi, active_bounds=False):
-    """The clipped trial point of one damped Gauss-Newton step (joint.py:149-162), float64."""
+    """The clipped trial point of one damped Gauss-Newton step (joint.py:149-162), float64.
+
+    active_bounds: a value on a bound whose step leaves the box is held for this step (identity row and
+    column, zero gradient)."""
     x, J, r = np.asarray(x, np.float64), np.asarray(J, np.float64), np.asarray(r, np.float64)
     lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
-    g, H = J.T @ r, J.T @ J
-    eye = np.eye(x.shape[0])
+    g = J.T @ r
+    H = J.T @ J
     if active_bounds:
-        keep = np.where(((x <= lo) & (g > 0)) | ((x >= hi) & (g < 0)), 0.0, 1.0)
-        H = H * keep[:, None] * keep[None, :] + np.diag(1.0 - keep)
-        g = g * keep
-    return np.clip(x + np.linalg.solve(H + float(lam) * (np.diag(np.diag(H)) + 1e-9 * eye), -g), lo, hi)
+        railed = ((x <= lo) & (g > 0)) | ((x >= hi) & (g < 0))
+        H[railed, :] = 0.0
+        H[:, railed] = 0.0
+        H[railed, railed] = 1.0
+        g = np.where(railed, 0.0, g)
+    A = H + float(lam) * np.diag(np.diag(H) + 1e-9)
+    return np.clip(x + np.linalg.solve(A, -g), lo, hi)
 
 
 class _Model:
@@ -193,15 +237,7 @@ class _Model:
         return Z, J
 
     def _run(self, q, ev, m0, tan_slots, seeds):
-        n = self.n
-        if tan_slots:
-            dm0 = np.stack(seeds, axis=1)
-            m, Jb = _engine.simulate_with_jac(q, ev, m0.astype(self.dt), tan_slots, [1.0] * len(tan_slots), self.mz,
-                                              dM_init=dm0, n_voxels=n, dtype=self.dt)
-        else:
-            m = _engine.simulate_states(q, ev, m0.astype(self.dt), n_voxels=n, dtype=self.dt)
-            Jb = np.zeros((n, m.shape[1], 0))
-        return np.asarray(m[..., self.mz], np.float64), np.asarray(Jb, np.float64)
+        return self._run_inner(q, ev, m0, tan_slots, seeds, self.n)
 
     def _run_spread(self, q, ev, m0, eff, tan_slots, seeds, tan_l):
         """Every sample's isochromats as extra voxels; weighted mean (spread.py:77-106)."""
@@ -241,6 +277,79 @@ class _Model:
         return np.asarray(m[..., self.mz], np.float64), np.asarray(Jb, np.float64)
 
 
+def _kernel_writes_every_slot(spec, spread):
+    """True when bmc_lm_fit can run the separable fit itself: its kernels write a scaled value into a
+    packed field, which is exactly a "value" slot of a SimParams field with no spread around it."""
+    return spread is None and all(s.mode == "value" and s.field not in PSEUDO for s in spec.slots)
+
+
+def _separable_on_device(params, m0, events, mz, spec, z, x0, free, n_iter, lam0, dt):
+    """Every sample's own LM in one bmc_lm_fit call (joint.py:359-374). Returns (x (n, L), summed cost)."""
+    from .independent import _lm_batched
+    mask = None if free.all() else free.astype(np.float64)
+    x, ctraj, _, _ = _lm_batched(params, events, m0, mz, spec, z, x0, int(n_iter), dt, lam0=lam0, free=mask,
+                                 want_cost=True)
+    cost = ctraj.sum(axis=0) if ctraj is not None else np.zeros(0)
+    return x, cost
+
+
+def _separable_on_host(model, z, x, lo, hi, free, n_iter, lam0):
+    """Every sample's own LM (lm.py:43-61 per sample), float64 steps around GPU model evaluations."""
+    mask = None if free.all() else free.astype(np.float64)
+    n, L = x.shape
+    damp = np.full(n, float(lam0))
+    eye = np.eye(L)
+    history = []
+    for _ in range(int(n_iter)):
+        Zm, Jl = model.eval(x)
+        r = Zm - z
+        c = np.einsum("no,no->n", r, r)
+        if mask is not None:
+            Jl = Jl * mask[:, None, :]
+        g = np.einsum("nop,no->np", Jl, r)
+        H = np.einsum("nop,noq->npq", Jl, Jl)
+        A = H + damp[:, None, None] * (H * eye + 1e-9 * eye)
+        if mask is not None:
+            A = A + (1.0 - mask)[:, :, None] * eye
+        trial = np.clip(x + np.linalg.solve(A, -g[..., None])[..., 0], lo, hi)
+        Zt, _ = model.eval(trial, want_jac=False)
+        ct = np.einsum("no,no->n", Zt - z, Zt - z)
+        accept = ct < c
+        x = np.where(accept[:, None], trial, x)
+        damp = damp * np.where(accept, 0.5, 3.0)
+        history.append(np.where(accept, ct, c))
+    return x, (np.asarray(history).sum(axis=1) if history else np.zeros(0))
+
+
+def _coupled(resid, jac, x, lo, hi, n_iter, lam0, active_bounds, keep_jacobian, stop_lam, callback):
+    """One damping over the joint vector (joint.py:375-416). Returns (x, cost after every step, NaN steps)."""
+    lam = float(lam0)
+    costs, n_nan = [], 0
+    reuse = None                     # (r, J) of the current point after a rejected step (keep_jacobian)
+    for step in range(int(n_iter)):
+        r, J = reuse if reuse is not None else (resid(x), jac(x))
+        reuse = None
+        c = float(r @ r)
+        if np.isfinite(J).all():
+            trial = _lm_trial(x, lam, J, r, lo, hi, active_bounds)
+            c_trial = float(np.sum(resid(trial) ** 2))
+            if c_trial < c:
+                x, c, lam = trial, c_trial, lam * 0.5
+            else:
+                lam *= 3.0
+                if keep_jacobian:
+                    reuse = (r, J)
+        else:                        # a Jacobian with NaN or inf: no step, harder damping, counted
+            n_nan += 1
+            lam *= 3.0
+        costs.append(c)
+        if callback is not None:
+            callback(step, c)
+        if stop_lam is not None and lam > stop_lam:
+            break
+    return x, np.asarray(costs), n_nan
+
+
 def fit_joint(experiment, spec, z_meas, n_iter=30, x0_phys=None, n_per_block=None, max_chunk=None, lam0=1e-2,
               callback=None, spread=None, block_jacobian="full", stop_lam=None, active_bounds=False,
               keep_jacobian=False, _internals=False):
@@ -254,8 +363,9 @@ def fit_joint(experiment, spec, z_meas, n_iter=30, x0_phys=None, n_per_block=Non
     params = experiment.params
     if np.ndim(params.water_r1) == 0:
         params = tile_params(params, n)
-    if np.shape(params.water_r1)[0] != n:
-        raise ValueError(f"params carry {np.shape(params.water_r1)[0]} samples, z_meas {n}")
+    n_given = np.shape(params.water_r1)[0]
+    if n_given != n:
+        raise ValueError(f"z_meas has {n} samples but the params describe {n_given}")
     params = SimParams(*[np.array(f) for f in params])
     m0 = np.asarray(experiment.m0)
     if m0.ndim == 1:
@@ -263,150 +373,104 @@ def fit_joint(experiment, spec, z_meas, n_iter=30, x0_phys=None, n_per_block=Non
     dt = np.asarray(params.water_r1).dtype
     z = z.astype(dt)
     events, mz = experiment.events, experiment.mz_loc
+
     unknown = [s.name for s in spec.slots if s.field not in PSEUDO and s.field not in SimParams._fields]
     if unknown:
-        raise ValueError(f"{unknown}: not a SimParams field nor one of {PSEUDO}")
-    l_sp = [l for l, s in enumerate(spec.slots) if s.field == B0_SPREAD]
-    if l_sp and spread is None:
-        raise ValueError(f"{[spec.slots[l].name for l in l_sp]}: b0_spread needs fit_joint(..., spread=B0Spread(...))")
-    if len(l_sp) > 1:
-        raise ValueError(f"{[spec.slots[l].name for l in l_sp]}: one b0_spread slot at most")
-    l_sp = l_sp[0] if l_sp else None
+        raise ValueError(f"{unknown} name neither a SimParams field nor one of {PSEUDO}")
+    spread_slots = [l for l, s in enumerate(spec.slots) if s.field == B0_SPREAD]
+    names = [spec.slots[l].name for l in spread_slots]
+    if spread_slots and spread is None:
+        raise ValueError(f"{names}: a b0_spread value needs fit_joint(..., spread=B0Spread(...)) to act on")
+    if len(spread_slots) > 1:
+        raise ValueError(f"{names}: at most one b0_spread value per fit")
+    l_sp = spread_slots[0] if spread_slots else None
     sig_held = None if spread is None else np.broadcast_to(np.asarray(spread.sigma_hz, np.float64), (n,))
     if block_jacobian not in ("full", "split"):
-        raise ValueError(f"block_jacobian {block_jacobian!r}; use 'full' or 'split'")
+        raise ValueError(f"block_jacobian must be 'full' or 'split', not {block_jacobian!r}")
     blocked = any(s.scope == "block" for s in spec.slots)
     if blocked and (not n_per_block or n_pts % n_per_block):
-        raise ValueError(f"a per-block value needs n_per_block dividing the {n_pts} offsets (got {n_per_block})")
-    nb = n_pts // n_per_block if blocked else 1
-    free, pos, n_loc, G, cols = _layout(spec, n, nb)
-    P = G.shape[2]
-    L = spec.n
+        raise ValueError(f"per-block values need n_per_block to divide the {n_pts} offsets; got {n_per_block}")
+    n_blocks = n_pts // n_per_block if blocked else 1
+
+    unk = _unknowns(spec, n, n_blocks)
+    free = unk.free
+    P, L = unk.n_joint, spec.n
     scales = spec.scales
     is_block = np.array([s.scope == "block" for s in spec.slots])
-    carried = _carried(params, m0, spec, sig_held).astype(dt).astype(np.float64)
+    carried = _held_values(params, m0, spec, sig_held).astype(dt).astype(np.float64)
     X0 = np.broadcast_to(spec.x0_phys, free.shape) if x0_phys is None else np.asarray(x0_phys, float)
     if X0.shape != free.shape:
-        raise ValueError(f"x0_phys has shape {X0.shape}, expected {free.shape} (samples, parameters)")
+        raise ValueError(f"x0_phys must have shape {free.shape} (samples, parameters), not {X0.shape}")
     X0 = np.where(free, X0, carried)
 
-    model = _Model(params, m0, events, mz, spec, free, carried, spread, l_sp, sig_held, blocked, nb, n_per_block,
-                   pos, n_loc)
-    locs = lambda x: np.einsum("ijp,p->ij", G, np.asarray(x, np.float64))
+    model = _Model(params, m0, events, mz, spec, free, carried, spread, l_sp, sig_held, blocked, n_blocks,
+                   n_per_block, unk.first, unk.n_local)
 
     def resid_all(x):
-        Zm, _ = model.eval(locs(x), want_jac=False)
+        Zm, _ = model.eval(unk.local(x), want_jac=False)
         return (Zm - z).reshape(-1)
 
     def jac_all(x):
-        _, Jl = model.eval(locs(x))
-        return np.einsum("inj,ijp->inp", Jl, G).reshape(-1, P)
+        _, Jl = model.eval(unk.local(x))
+        return unk.joint_jacobian(Jl)
 
-    x_start = np.asarray([X0[who[0], l] / scales[l] for l, who, _ in cols], np.float64)
+    # a shared or per-block value starts at its first member's entry
+    x_start = np.array([X0[members[0], l] / scales[l] for l, members, _ in unk.columns], np.float64)
     if _internals:
-        return dict(resid=resid_all, jac=jac_all, x0=x_start, cols=cols, model=model, locs=locs)
+        return dict(resid=resid_all, jac=jac_all, x0=x_start, cols=unk.columns, model=model, locs=unk.local)
 
-    coupled = any(s.scope in ("shared", "block") for s in spec.slots)
+    coupled = any(s.scope != "sample" for s in spec.slots)
     if not coupled and (stop_lam is not None or active_bounds or keep_jacobian):
-        raise ValueError("stop_lam, active_bounds and keep_jacobian apply to a coupled fit (a shared or "
-                         "per-block value); without one every sample has its own damping, as in `fit`")
+        raise ValueError("stop_lam, active_bounds and keep_jacobian only act on a coupled fit (one with a "
+                         "shared or per-block value); a separable fit damps every sample alone, as fit does")
     nan_steps = 0
     if not coupled:
-        # block diagonal: every sample alone with its own damping -- fit's algorithm (joint.py:359-374)
-        hold = None if free.all() else free.astype(np.float64)
-        lo = spec.lo_phys / scales
-        hi = spec.hi_phys / scales
-        x = (X0 / scales).astype(dt).astype(np.float64)
-        lam = np.full(n, float(lam0))
-        eye = np.eye(L)
-        cost_traj = []
-        for _ in range(int(n_iter)):
-            Zm, Jl = model.eval(x)
-            r = Zm - z
-            c = np.sum(r * r, axis=1)
-            if hold is not None:
-                Jl = Jl * hold[:, None, :]
-            g = np.einsum("nop,no->np", Jl, r)
-            H = np.einsum("nop,noq->npq", Jl, Jl)
-            A = H + lam[:, None, None] * (H * eye + 1e-9 * eye)
-            if hold is not None:
-                A = A + (1.0 - hold)[:, :, None] * eye
-            delta = np.linalg.solve(A, -g[..., None])[..., 0]
-            xn = np.clip(x + delta, lo, hi)
-            Zn, _ = model.eval(xn, want_jac=False)
-            cn = np.sum((Zn - z) ** 2, axis=1)
-            better = cn < c
-            x = np.where(better[:, None], xn, x)
-            lam = np.where(better, lam * 0.5, lam * 3.0)
-            cost_traj.append(np.where(better, cn, c))
-        loc = x
-        cost = np.asarray(cost_traj, float).sum(axis=1) if cost_traj else np.zeros(0)
+        x0 = (X0 / scales).astype(dt).astype(np.float64)
+        if _kernel_writes_every_slot(spec, spread):
+            loc, cost = _separable_on_device(params, m0, events, mz, spec, z, x0, free, n_iter, lam0, dt)
+        else:
+            loc, cost = _separable_on_host(model, z, x0, spec.lo_phys / scales, spec.hi_phys / scales, free,
+                                           n_iter, lam0)
+        col_slot = col_scale = x = None
     else:
-        # coupled: one joint step over every free value, one damping (joint.py:375-416)
-        col_slot = np.array([l for l, _, _ in cols], int)
+        col_slot = np.array([l for l, _, _ in unk.columns], int)
         col_scale = scales[col_slot]
-        lo = spec.lo_phys[col_slot] / col_scale
-        hi = spec.hi_phys[col_slot] / col_scale
-        x = x_start.astype(dt).astype(np.float64)
-        lam, cost = float(lam0), []
-        kept = None
-        for it in range(int(n_iter)):
-            if kept is None:
-                r, J = resid_all(x), jac_all(x)
-                finite = bool(np.all(np.isfinite(J)))
-            else:
-                (r, J), finite = kept, True
-            if finite:
-                c = float(r @ r)
-                xn = _lm_trial(x, lam, J, r, lo, hi, active_bounds)
-                cn = float(np.sum(resid_all(xn) ** 2))
-                better = cn < c
-                if better:
-                    x, c = xn, cn
-                lam = lam * 0.5 if better else lam * 3.0
-                cost.append(c)
-                kept = (r, J) if keep_jacobian and not better else None
-            else:
-                nan_steps += 1
-                lam = lam * 3.0
-                cost.append(float(r @ r))
-            if callback is not None:
-                callback(it, cost[-1])
-            if stop_lam is not None and lam > stop_lam:
-                break
-        loc = locs(x)
-        cost = np.asarray(cost)
+        x, cost, nan_steps = _coupled(resid_all, jac_all, x_start.astype(dt).astype(np.float64),
+                                      spec.lo_phys[col_slot] / col_scale, spec.hi_phys[col_slot] / col_scale,
+                                      n_iter, lam0, active_bounds, keep_jacobian, stop_lam, callback)
+        loc = unk.local(x)
 
-    loc_np = np.asarray(loc, float)
-    X = np.where(free, loc_np[:, pos] * scales, carried)
-    z_fit, Jl = model.eval(loc_np)
-    r_all = z_fit - np.asarray(z, float)
+    loc = np.asarray(loc, float)
+    X = np.where(free, loc[:, unk.first] * scales, carried)
+    z_fit, Jl = model.eval(loc)
+    misfit = z_fit - np.asarray(z, float)
     sd = np.full(free.shape, np.nan)
     corr = np.full(free.shape + (L,), np.nan)
     blocks = {}
     if not coupled:
-        for i in range(n):
-            use = np.flatnonzero(free[i])
-            if use.size:
-                Ju = Jl[i][:, pos[use]]
-                s2 = float(r_all[i] @ r_all[i]) / max(r_all.shape[1] - use.size, 1)
-                sd[i, use], corr[i][np.ix_(use, use)] = _cov_to_sd_corr(np.linalg.pinv(Ju.T @ Ju) * s2, scales[use])
+        # misfit-scaled error bars per sample: s^2 = cost_i / (points - its free values)
+        for i in np.flatnonzero(free.any(axis=1)):
+            mine = np.flatnonzero(free[i])
+            Ji = Jl[i][:, unk.first[mine]]
+            s2 = float(misfit[i] @ misfit[i]) / max(n_pts - mine.size, 1)
+            sd[i, mine], corr[i][np.ix_(mine, mine)] = _errors(np.linalg.pinv(Ji.T @ Ji) * s2, scales[mine])
     else:
-        J = np.einsum("inj,ijp->inp", Jl, G).reshape(-1, P)
-        s2 = float(np.sum(r_all ** 2)) / max(r_all.size - P, 1)
-        sd_col, corr_col = _cov_to_sd_corr(np.linalg.pinv(J.T @ J) * s2, col_scale)
+        # one covariance over the joint vector, s^2 = cost / (all points - joint values)
+        Jj = unk.joint_jacobian(Jl)
+        s2 = float(np.sum(misfit ** 2)) / max(misfit.size - P, 1)
+        sd_j, corr_j = _errors(np.linalg.pinv(Jj.T @ Jj) * s2, col_scale)
         for i in range(n):
-            ps = [p for p, (l, who, b) in enumerate(cols) if i in who and b < 0]
-            ls = col_slot[ps]
-            sd[i, ls] = sd_col[ps]
-            corr[i][np.ix_(ls, ls)] = corr_col[np.ix_(ps, ps)]
+            own = [p for p, (_, members, b) in enumerate(unk.columns) if b < 0 and i in members]
+            slots_i = col_slot[own]
+            sd[i, slots_i] = sd_j[own]
+            corr[i][np.ix_(slots_i, slots_i)] = corr_j[np.ix_(own, own)]
         for l in np.flatnonzero(is_block):
-            ps = [p for p, (ll, _, b) in enumerate(cols) if ll == l]
-            xb = np.asarray(x, float)[ps] * scales[l]
-            blocks[spec.slots[l].name] = dict(x=xb, sd=sd_col[ps], at_bounds=(
-                np.isclose(xb, spec.slots[l].lo) | np.isclose(xb, spec.slots[l].hi)))
+            s = spec.slots[l]
+            own = [p for p, col in enumerate(unk.columns) if col[0] == l]
+            xb = np.asarray(x, float)[own] * scales[l]
+            blocks[s.name] = dict(x=xb, sd=sd_j[own], at_bounds=np.isclose(xb, s.lo) | np.isclose(xb, s.hi))
+    # per-block values are reported in `blocks`; their column of x is NaN
     X[:, is_block] = np.nan
     X0 = np.where(is_block[None, :], np.nan, X0)
-    free = free & ~is_block[None, :]
-    return FitResult(X, z_fit, np.asarray(z, float), spec, n_per_block, free=free, sd=sd, corr=corr, x0=X0, cost=cost,
-                     nan_steps=nan_steps, blocks=blocks or None)
+    return FitResult(X, z_fit, np.asarray(z, float), spec, n_per_block, free=free & ~is_block[None, :], sd=sd,
+                     corr=corr, x0=X0, cost=cost, nan_steps=nan_steps, blocks=blocks or None)

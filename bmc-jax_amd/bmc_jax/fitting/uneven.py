@@ -50,10 +50,12 @@ def _bind(L):
     L.bmc_ubatch_sizes.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                    ctypes.POINTER(ctypes.c_int64)]
     L.bmc_ubatch_simulate.argtypes = [vp, vp, vp, vp, vp]
+    L.bmc_ubatch_simulate_ex.argtypes = [vp, vp, vp, vp, ctypes.c_uint, vp]
     L.bmc_ubatch_lm_fit.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint, ctypes.c_double,
                                     ctypes.c_double, vp, vp, vp]
     L.bmc_ubatch_destroy.argtypes = [vp]
     for n in ("bmc_ubatch_create", "bmc_ubatch_configure", "bmc_ubatch_sizes", "bmc_ubatch_simulate",
+              "bmc_ubatch_simulate_ex",
               "bmc_ubatch_lm_fit", "bmc_ubatch_destroy"):
         getattr(L, n).restype = ctypes.c_int
     _BOUND = True
@@ -148,16 +150,18 @@ class UnevenBatch:
         return np.concatenate([p.spec.to_scaled(p.spec.x0_phys if p.x0_phys is None else p.x0_phys)
                                for p in self.problems])
 
-    def simulate(self, x_scaled, jacobian=True):
-        """(z [R], J ragged [sum n_rows P] or None) as float64 device tensors at x (scaled, concatenated)."""
+    def simulate(self, x_scaled, jacobian=True, forward_jacobian=False):
+        """(z [R], J ragged [sum n_rows P] or None) as float64 device tensors at x (scaled, concatenated).
+        forward_jacobian: the forward-mode sweep even for single-event rows."""
         torch = _native.torch_cuda()
         x = torch.as_tensor(np.asarray(x_scaled, np.float64), device="cuda")
         z = torch.empty(self.R, dtype=torch.float64, device="cuda")
         J = torch.empty(max(self.n_jacobian, 1), dtype=torch.float64, device="cuda") if jacobian else None
-        _native.check(_native.lib().bmc_ubatch_simulate(self._h, ctypes.c_void_p(x.data_ptr()),
-                                                         ctypes.c_void_p(z.data_ptr()),
-                                                         ctypes.c_void_p(J.data_ptr()) if J is not None else None,
-                                                         _native.stream_ptr()), "bmc_ubatch_simulate")
+        flags = _native.SIM_FORWARD_JACOBIAN if forward_jacobian else 0
+        _native.check(_native.lib().bmc_ubatch_simulate_ex(self._h, ctypes.c_void_p(x.data_ptr()),
+                                                            ctypes.c_void_p(z.data_ptr()),
+                                                            ctypes.c_void_p(J.data_ptr()) if J is not None else None,
+                                                            flags, _native.stream_ptr()), "bmc_ubatch_simulate_ex")
         return z, J
 
     def jacobian_of(self, J, i):
@@ -167,14 +171,15 @@ class UnevenBatch:
         return Jh[off:off + self.n_rows[i] * self.n_params[i]].reshape(self.n_rows[i], self.n_params[i])
 
     def lm_fit(self, x0_scaled=None, n_iter=30, lam0=1e-2, early_exit=False, early_ftol=1e-10, early_rms=0.0,
-               graph=True):
+               graph=True, forward_jacobian=False):
         """bmc_ubatch_lm_fit. Returns (x scaled [sum P], final cost [n], iterations run [n]) as NumPy."""
         torch = _native.torch_cuda()
         x = torch.as_tensor(np.asarray(self.x0_scaled() if x0_scaled is None else x0_scaled, np.float64),
                             device="cuda").clone()
         cost = torch.empty(self.n, dtype=torch.float64, device="cuda")
         nd = torch.empty(self.n, dtype=torch.int32, device="cuda")
-        flags = (_native.LM_EARLY_EXIT if early_exit else 0) | (_native.LM_GRAPH if graph else 0)
+        flags = ((_native.LM_EARLY_EXIT if early_exit else 0) | (_native.LM_GRAPH if graph else 0) |
+                 (_native.LM_FORWARD_JACOBIAN if forward_jacobian else 0))
         _native.check(_native.lib().bmc_ubatch_lm_fit(self._h, ctypes.c_void_p(self.z_meas.data_ptr()),
                                                        ctypes.c_void_p(x.data_ptr()), int(n_iter), float(lam0),
                                                        flags, float(early_ftol), float(early_rms),

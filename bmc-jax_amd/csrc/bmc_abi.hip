@@ -492,10 +492,14 @@ int tangents_per_pass(int P) { return P <= 0 ? 0 : (P < BMC_TANGENTS_PER_PASS ? 
 
 extern "C" int bmc_tangents_per_pass(int n_tangents) { return tangents_per_pass(n_tangents); }
 
-extern "C" int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
-                            const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
-                            const double* tangent_scale, int jac_component, void* m_out_dev, void* jac_out_dev,
-                            void* stream) {
+extern "C" int bmc_simulate_ex(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                               const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
+                               const double* tangent_scale, int jac_component, void* m_out_dev, void* jac_out_dev,
+                               unsigned flags, void* stream) {
+  if (flags & ~BMC_SIM_FORWARD_JACOBIAN) {
+    set_error("bmc_simulate_ex: unknown flag bits");
+    return BMC_EINVAL;
+  }
   SimArgs a;
   int rc = fill_sim_args(plan, n_voxels, n_cest, dtype, params_dev, m0_dev, dm0_dev, n_tangents, tangent_slot,
                          tangent_scale, jac_component, a);
@@ -508,8 +512,17 @@ extern "C" int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int 
   a.PT = tangents_per_pass(want_tan);
   a.n_pass = want_tan ? (want_tan + a.PT - 1) / a.PT : 1;
   if (!want_tan) a.P = n_tangents;  // seeds unused
-  if (want_tan && use_cwj(plan->p, a)) return launch_cwj(n_cest, a, plan->p.d_live, (hipStream_t)stream);
+  if (want_tan && use_cwj(plan->p, a, (flags & BMC_SIM_FORWARD_JACOBIAN) != 0))
+    return launch_cwj(n_cest, a, plan->p.d_live, (hipStream_t)stream);
   return launch_sim(n_cest, a, (hipStream_t)stream);
+}
+
+extern "C" int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                            const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
+                            const double* tangent_scale, int jac_component, void* m_out_dev, void* jac_out_dev,
+                            void* stream) {
+  return bmc_simulate_ex(plan, n_voxels, n_cest, dtype, params_dev, m0_dev, dm0_dev, n_tangents, tangent_slot,
+                         tangent_scale, jac_component, m_out_dev, jac_out_dev, 0u, stream);
 }
 
 extern "C" int bmc_simulate_trajectory(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,

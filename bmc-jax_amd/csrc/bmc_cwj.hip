@@ -54,7 +54,7 @@ __device__ __forceinline__ void cwj_team(const SimArgs& a, const int32_t* row_li
   for (int i = lane; i < I::stride; i += TEAM) {
     double val = load_real(a.params, (long)v * a.stride + i, a.dtype);
     for (int w = 0; w < a.n_write; w++)
-      if (a.w_slot[w] == i) {  // write_params (fitting/params.py:100-112)
+      if (a.w_slot[w] == i && (!a.w_keep || a.w_keep[(long)v * a.n_write + w] != 0.0)) {  // write_params (fitting/params.py:100-112)
         double xv = a.xw[(long)v * a.n_write + w] * a.w_scale[w];
         val = (a.dtype == BMC_F32) ? (double)(float)xv : xv;
       }
@@ -157,14 +157,9 @@ static int launch_cwj_nc(const SimArgs& a, const int32_t* row_live, hipStream_t 
   return BMC_OK;
 }
 
-bool forward_jacobian_forced() {
-  const char* f = getenv("BMC_FORWARD_JACOBIAN");
-  return f && f[0] && f[0] != '0';
-}
-
-bool use_cwj(const Plan& p, const SimArgs& a) {
-  if (!p.cw_rows || !p.d_live || a.dm0 || a.traj || a.P <= 0 || !a.jac_out) return false;
-  return !forward_jacobian_forced();
+bool use_cwj(const Plan& p, const SimArgs& a, bool forward_mode) {
+  if (forward_mode || !p.cw_rows || !p.d_live || a.dm0 || a.traj || a.P <= 0 || !a.jac_out) return false;
+  return true;
 }
 
 int launch_cwj(int nc, const SimArgs& a, const int32_t* row_live, hipStream_t stream) {

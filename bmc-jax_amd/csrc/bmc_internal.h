@@ -67,6 +67,9 @@ struct SimArgs {
   const void* dm0;        // [V, P, N] dtype or null
   // values written into slots before simulating (fit_batch's write_params): [V, n_write] float64
   const double* xw;
+  // [V, n_write] float64 or null: where 0, the slot keeps the record's own value (a held entry of a
+  // partly free fit, whose model uses the carried value exactly: fitting/joint.py:292-294)
+  const double* w_keep;
   int n_write;
   int w_slot[kMaxTan];
   double w_scale[kMaxTan];
@@ -109,10 +112,10 @@ void set_error(const std::string& msg);
 int tan_kind_of_slot(int slot, int nc, int* idx);
 int launch_sim(int nc, const SimArgs& a, hipStream_t stream);
 // The Jacobian of single-event rows by reverse accumulation through the squarings (bmc_cwj.hip):
-// use_cwj says whether a tangent launch of this plan can take it (no M0 seeds, no trajectory; the
-// environment variable BMC_FORWARD_JACOBIAN=1 keeps the forward-mode kernel), launch_cwj runs it.
-bool use_cwj(const Plan& p, const SimArgs& a);
-bool forward_jacobian_forced();
+// use_cwj says whether a tangent launch of this plan can take it (no M0 seeds, no trajectory, and the
+// caller did not ask for forward mode with BMC_SIM_FORWARD_JACOBIAN / BMC_LM_FORWARD_JACOBIAN),
+// launch_cwj runs it.
+bool use_cwj(const Plan& p, const SimArgs& a, bool forward_mode);
 int launch_cwj(int nc, const SimArgs& a, const int32_t* row_live, hipStream_t stream);
 // Tangents carried per simulate pass for P tangents (register budget of the kernel).
 int tangents_per_pass(int P);

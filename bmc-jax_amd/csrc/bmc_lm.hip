@@ -322,6 +322,7 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
 
   // launch 1: sim with tangents at x
   sj.xw = x_dev;
+  sj.w_keep = free_dev;  // a held value is not written: the record's own number stays
   sj.n_write = P;
   for (int p = 0; p < P; p++) {
     sj.w_slot[p] = slot[p];
@@ -378,7 +379,7 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   ac.s = st;
 
   // single-event rows take the Jacobian by reverse accumulation through the squarings
-  const bool cwj = use_cwj(plan->p, sj);
+  const bool cwj = use_cwj(plan->p, sj, (flags & BMC_LM_FORWARD_JACOBIAN) != 0);
   hipLaunchKernelGGL(init_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, st, V, lam0);
   BMC_HIP_CHECK(hipGetLastError());
   if ((flags & BMC_LM_START_CHECK) && (flags & BMC_LM_EARLY_EXIT)) {

@@ -77,7 +77,7 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   for (int i = lane; i < I::stride; i += TEAM) {
     double val = load_real(a.params, (long)v * a.stride + i, a.dtype);
     for (int w = 0; w < a.n_write; w++)
-      if (a.w_slot[w] == i) {
+      if (a.w_slot[w] == i && (!a.w_keep || a.w_keep[(long)v * a.n_write + w] != 0.0)) {
         double xv = a.xw[(long)v * a.n_write + w] * a.w_scale[w];
         val = (a.dtype == BMC_F32) ? (double)(float)xv : xv;
       }

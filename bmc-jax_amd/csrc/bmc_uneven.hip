@@ -535,7 +535,7 @@ int upload_chunks(bmc_ubatch* b, const std::vector<int2>& chunks, int2** dc) {
 }
 
 USimArgs sim_args(const bmc_ubatch* b, bool tangents, const double* xw, double* z_out, double* jac_out,
-                  const int32_t* active) {
+                  const int32_t* active, bool forward_mode) {
   USimArgs a;
   memset(&a, 0, sizeof(a));
   const Plan& p = b->plan->p;
@@ -544,7 +544,7 @@ USimArgs sim_args(const bmc_ubatch* b, bool tangents, const double* xw, double* 
   a.row_ptr = p.d_row_ptr;
   a.T = p.T;
   // single-event rows: the Jacobian sweep takes one item per row and every tangent at once
-  const bool reverse = tangents && p.cw_rows && p.d_live && !forward_jacobian_forced();
+  const bool reverse = tangents && p.cw_rows && p.d_live && !forward_mode;
   a.reset_after_adc = p.reset_after_adc;
   a.dtype = b->dtype;
   a.PT = tangents ? b->PT : 0;
@@ -725,14 +725,20 @@ extern "C" int bmc_ubatch_sizes(const bmc_ubatch* b, int* n_rows, int* n_values,
   return BMC_OK;
 }
 
-extern "C" int bmc_ubatch_simulate(bmc_ubatch* b, const double* x_dev, double* z_out_dev, double* jac_out_dev,
-                                   void* stream) {
-  if (!b || !x_dev || !z_out_dev) {
-    set_error("bmc_ubatch_simulate: null argument");
+extern "C" int bmc_ubatch_simulate_ex(bmc_ubatch* b, const double* x_dev, double* z_out_dev, double* jac_out_dev,
+                                      unsigned flags, void* stream) {
+  if (!b || !x_dev || !z_out_dev || (flags & ~BMC_SIM_FORWARD_JACOBIAN)) {
+    set_error("bmc_ubatch_simulate: null argument or unknown flag bits");
     return BMC_EINVAL;
   }
-  const USimArgs a = sim_args(b, jac_out_dev != nullptr, x_dev, z_out_dev, jac_out_dev, nullptr);
+  const USimArgs a = sim_args(b, jac_out_dev != nullptr, x_dev, z_out_dev, jac_out_dev, nullptr,
+                              (flags & BMC_SIM_FORWARD_JACOBIAN) != 0);
   return launch_usim(b, a, (hipStream_t)stream);
+}
+
+extern "C" int bmc_ubatch_simulate(bmc_ubatch* b, const double* x_dev, double* z_out_dev, double* jac_out_dev,
+                                   void* stream) {
+  return bmc_ubatch_simulate_ex(b, x_dev, z_out_dev, jac_out_dev, 0u, stream);
 }
 
 extern "C" int bmc_ubatch_lm_fit(bmc_ubatch* b, const void* z_meas_dev, double* x_dev, int n_iter, double lam0,
@@ -773,8 +779,9 @@ extern "C" int bmc_ubatch_lm_fit(bmc_ubatch* b, const void* z_meas_dev, double* 
   L.n_done = n_iter_done_dev;
   L.ftol = early_ftol;
   L.rms_tol = early_rms;
-  const USimArgs sj = sim_args(b, true, x_dev, z, J, active);
-  const USimArgs sp = sim_args(b, false, xt, zt, nullptr, active);
+  const bool fwd = (flags & BMC_LM_FORWARD_JACOBIAN) != 0;
+  const USimArgs sj = sim_args(b, true, x_dev, z, J, active, fwd);
+  const USimArgs sp = sim_args(b, false, xt, zt, nullptr, active, fwd);
 
   hipStream_t st = (hipStream_t)stream;
   const bool use_graph = (flags & BMC_LM_GRAPH) != 0;

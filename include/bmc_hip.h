@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define BMC_ABI_VERSION 1
+#define BMC_ABI_VERSION 2
 
 #define BMC_F32 0
 #define BMC_F64 1
@@ -120,13 +120,22 @@ int bmc_plan_program(const bmc_plan* plan, int64_t capacity, int32_t* ops_out, i
  * (simulation/expm.py:83) makes the state NaN until the next ADC reset.
  * When every row of the plan propagates at most one event (CW saturation rows: no spoiler or gradient,
  * a readout only as a row's last event) and dm0_dev is NULL, the tangents come from the reverse-
- * accumulation kernel instead (same outputs, to rounding); BMC_FORWARD_JACOBIAN=1 in the environment
- * keeps forward mode. bmc_lm_fit takes the same route for its Jacobian sweeps.
+ * accumulation kernel instead (same outputs, to rounding). bmc_lm_fit takes the same route for its
+ * Jacobian sweeps. The library reads no environment: the mode is chosen per call (bmc_simulate_ex).
  */
 int bmc_simulate(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
                  const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
                  const double* tangent_scale, int jac_component, void* m_out_dev, void* jac_out_dev,
                  void* stream);
+/* bmc_simulate with a flags word. BMC_SIM_FORWARD_JACOBIAN: take the tangents in forward mode (what
+ * jax.jacfwd computes, fitting/lm.py:39) even where the reverse-accumulation kernel applies. Same
+ * outputs to rounding; the bit exists to compare the two routes and to time them. Unknown bits:
+ * BMC_EINVAL. bmc_simulate is bmc_simulate_ex with flags 0. */
+#define BMC_SIM_FORWARD_JACOBIAN 1u
+int bmc_simulate_ex(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                    const void* m0_dev, const void* dm0_dev, int n_tangents, const int32_t* tangent_slot,
+                    const double* tangent_scale, int jac_component, void* m_out_dev, void* jac_out_dev,
+                    unsigned flags, void* stream);
 
 /*
  * The state after every event: simulate_sequence's return_full_trajectory=True (simulate.py:115-120,
@@ -142,24 +151,29 @@ int bmc_simulate_trajectory(const bmc_plan* plan, int n_voxels, int n_cest, int 
  * simulation at the trial point, accept iff strictly better (lam x 0.5) else reject (lam x 3).
  *
  * Replaces: levenberg_marquardt vmapped by fit_batch (fitting/lm.py:20-67, independent.py:118-128) and
- * fit_joint's non-coupled path (joint.py:359-374).
+ * fit_joint's non-coupled path (joint.py:359-374) when every slot is a plain value of a packed field
+ * (held entries through free_dev; factor, offset, start-state and B0-spread slots have no kernel form).
  *
  *   x_dev        [V, P] float64 in/out       scaled start values in, fitted scaled values out
  *   lo, hi       [P] host float64            bounds, scaled units
  *   z_meas_dev   [V, O] dtype                measured Z (residual r = Z_sim[:, jac_component] - z_meas)
- *   free_dev     [V, P] float64 or NULL      1 free / 0 held (NULL = all free, arithmetic unchanged)
+ *   free_dev     [V, P] float64 or NULL      1 free / 0 held (NULL = all free, arithmetic unchanged); a
+ *                                            held value is never written into the record, so the
+ *                                            model keeps the record's own number for it exactly
  *   cost_traj_dev[V, n_iter] float64 or NULL cost after every iteration (the reference's C_traj)
  *   x_traj_dev   [V, n_iter, P] float64 or NULL
  *   flags        BMC_LM_EARLY_EXIT: stop a problem once it has converged (never the reference's
  *                behaviour, which runs every iteration); BMC_LM_GRAPH: replay the iteration as a
  *                captured hipGraph; BMC_LM_START_CHECK (with BMC_LM_EARLY_EXIT): simulate every start
  *                first and stop, before any Jacobian, the problems whose start rms is at most
- *                early_rms (they return x0 after 0 iterations).
+ *                early_rms (they return x0 after 0 iterations); BMC_LM_FORWARD_JACOBIAN: Jacobian sweeps
+ *                in forward mode, as BMC_SIM_FORWARD_JACOBIAN.
  *   n_iter_done_dev [V] int32 or NULL        iterations each problem ran
  */
 #define BMC_LM_EARLY_EXIT 1u
 #define BMC_LM_GRAPH 2u
 #define BMC_LM_START_CHECK 4u
+#define BMC_LM_FORWARD_JACOBIAN 8u
 int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
                const void* m0_dev, int n_params, const int32_t* slot, const double* scale, const double* lo,
                const double* hi, int jac_component, const void* z_meas_dev, double* x_dev, int n_iter,
@@ -221,6 +235,9 @@ int bmc_ubatch_create(const bmc_plan* plan, int n_problems, int dtype, const int
 int bmc_ubatch_configure(bmc_ubatch* batch, int tangents_per_pass, int grid, int order);
 int bmc_ubatch_sizes(const bmc_ubatch* batch, int* n_rows, int* n_values, int64_t* n_jacobian);
 int bmc_ubatch_simulate(bmc_ubatch* batch, const double* x_dev, double* z_out_dev, double* jac_out_dev, void* stream);
+/* bmc_ubatch_simulate with a flags word (BMC_SIM_FORWARD_JACOBIAN as for bmc_simulate_ex). */
+int bmc_ubatch_simulate_ex(bmc_ubatch* batch, const double* x_dev, double* z_out_dev, double* jac_out_dev,
+                           unsigned flags, void* stream);
 int bmc_ubatch_lm_fit(bmc_ubatch* batch, const void* z_meas_dev, double* x_dev, int n_iter, double lam0,
                       unsigned flags, double early_ftol, double early_rms, double* cost_dev, int32_t* n_iter_done_dev,
                       void* stream);

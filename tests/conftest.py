@@ -4,6 +4,8 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.append(ROOT)                # bench.py's workload generators
 for sub in ("bmc-jax_amd", "oracle"):
     p = os.path.join(ROOT, sub)
     if p not in sys.path:

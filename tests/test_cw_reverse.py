@@ -5,7 +5,6 @@ For E = T^n, T = exp(Y), Y = X / n, n = 2^s and Z = e_z^T E v, the kernel comput
 dZ/dp = <dT_p, G_n> with G_1 = e_z v^T and G_2n = G_n E_k^T + E_k^T G_n (E_k = T^n at that level),
 keeping only the rows of G above the augmented one. Here the same recursion in NumPy must give
 e_z^T L(X, dX_p) v, L scipy's Frechet derivative of exp at the unscaled X (an independent route)."""
-import os
 
 import numpy as np
 import pytest
@@ -77,7 +76,7 @@ def test_reverse_accumulation_matches_the_frechet_derivative(s):
 @pytest.mark.parametrize("nc", [0, 1, 2, 3, 4])
 def test_reverse_jacobian_matches_forward_mode(nc):
     """bmc_simulate on CW rows: the reverse kernel (default) against the forward-mode kernel
-    (BMC_FORWARD_JACOBIAN=1), every tangent kind that a CW row can carry, float64."""
+    (bmc_simulate_ex with BMC_SIM_FORWARD_JACOBIAN), every tangent kind that a CW row can carry, float64."""
     import torch
     from bmc_jax import _native as nat
     nat.torch_cuda()
@@ -97,13 +96,9 @@ def test_reverse_jacobian_matches_forward_mode(nc):
     jc = 2 * (1 + nc)
     out = {}
     for mode in ("reverse", "forward"):
-        if mode == "forward":
-            os.environ["BMC_FORWARD_JACOBIAN"] = "1"
-        try:
-            m, J = nat.simulate(plan, prm, nc, nat.BMC_F64, m0, slots=slots, scales=[1.0] * len(slots), jc=jc)
-            torch.cuda.synchronize()
-        finally:
-            os.environ.pop("BMC_FORWARD_JACOBIAN", None)
+        m, J = nat.simulate(plan, prm, nc, nat.BMC_F64, m0, slots=slots, scales=[1.0] * len(slots), jc=jc,
+                            forward_jacobian=(mode == "forward"))
+        torch.cuda.synchronize()
         out[mode] = (m.cpu().numpy(), J.cpu().numpy())
     (mr, Jr), (mf, Jf) = out["reverse"], out["forward"]
     # the kernels split exp(X) differently (Taylor degree cap 18 with more squarings against 30):
@@ -160,13 +155,9 @@ def test_reverse_kernel_edge_rows(nc):
     m0 = nat.device_array(m0s, torch.float64)
     out = {}
     for mode in ("reverse", "forward"):
-        if mode == "forward":
-            os.environ["BMC_FORWARD_JACOBIAN"] = "1"
-        try:
-            m, J = nat.simulate(plan, prm, nc, nat.BMC_F64, m0, slots=slots, scales=[1.0] * 3, jc=2 * (1 + nc))
-            torch.cuda.synchronize()
-        finally:
-            os.environ.pop("BMC_FORWARD_JACOBIAN", None)
+        m, J = nat.simulate(plan, prm, nc, nat.BMC_F64, m0, slots=slots, scales=[1.0] * 3, jc=2 * (1 + nc),
+                            forward_jacobian=(mode == "forward"))
+        torch.cuda.synchronize()
         out[mode] = (m.cpu().numpy(), J.cpu().numpy())
     (mr, Jr), (mf, Jf) = out["reverse"], out["forward"]
     np.testing.assert_array_equal(np.isnan(mr), np.isnan(mf))

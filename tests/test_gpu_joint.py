@@ -10,6 +10,8 @@ a truth the test builds itself (data simulated by the kernel, then fitted back).
 import numpy as np
 import pytest
 
+import bmc_oracle as O
+
 pytestmark = pytest.mark.gpu
 
 PPM = np.linspace(-6.0, 6.0, 41)
@@ -295,3 +297,98 @@ def test_the_joint_jacobian_is_the_derivative(api, specs, amide, case):
         settings.use_x64(False)
     rel = np.linalg.norm(J - Jn, axis=0) / np.maximum(np.linalg.norm(Jn, axis=0), 1e-30)
     assert rel.max() < 1e-5, f"{case}: columns {np.flatnonzero(rel >= 1e-5)} off by {rel.max():.1e}"
+
+
+MODEL_CASES = {
+    "factor + start state value": (["K", "F", "R2_FACTOR", "M_SHARED"], None, False),
+    "start state in some samples": (["K", "F", "M_SOME"], None, False),
+    "gaussian spread": (["K", "F", "SIGMA"], ("gauss", 3), False),
+    "per-block factor and offset": (["K", "F", "B1F", "DB0"], None, True),
+    "lorentzian spread + block factor": (["K", "F", "B1F", "SIGMA"], ("lorentz", 3), True),
+}
+
+
+def _record(sp, i):
+    """Sample i of a SimParams batch as an oracle record (float64 values, int lineshape)."""
+    out = {}
+    for f in sp._fields:
+        a = np.asarray(getattr(sp, f))
+        v = a[i] if a.ndim else a
+        out[f] = int(v) if f == "mt_lineshape" else np.array(v, np.float64)
+    return out
+
+
+def _spread_nodes(shape, nodes, cut=20.0):
+    """Probabilists' Gauss-Hermite, or Gauss-Legendre mapped by tan onto a Lorentzian cut at +-cut
+    (spread.py:67-74), from scipy's root tables, weights normalised to 1."""
+    from scipy.special import roots_hermitenorm, roots_legendre
+    if shape == "gauss":
+        u, w = roots_hermitenorm(nodes)
+    else:
+        t, w = roots_legendre(nodes)
+        u = np.tan(t * np.arctan(cut))
+    return u, w / w.sum()
+
+
+@pytest.mark.parametrize("case", list(MODEL_CASES))
+def test_the_joint_model_is_the_oracle(api, specs, amide, case):
+    """The joint model (every slot mode applied per sample and block, the start state rebuilt or scaled,
+    the B0 spread as weighted isochromats) against the float64 C oracle run on parameters this test
+    builds itself from the joint vector (joint.py:120-138, spread.py:77-106)."""
+    import oracle_c as OC
+    from bmc_jax import settings
+    p, jp, true, events, m0, z = amide
+    F, S = api["F"], api["S"]
+    names, sp, blocked = MODEL_CASES[case]
+    n, n_pts = z.shape
+    npb = PPM.size
+    settings.use_x64(True)
+    try:
+        t64 = type(true)(*[np.asarray(a, np.float64) if np.asarray(a).dtype.kind == "f" else np.asarray(a)
+                           for a in true])
+        spread = None if sp is None else S.B0Spread(sigma_hz=7.0, shape=sp[0], nodes=sp[1])
+        spec = F.FitSpec([specs[k] for k in names])
+        d = F.fit_joint(_exp(api, t64, events, m0.astype(np.float64), p.mz_loc), spec, z.astype(np.float64),
+                        spread=spread, n_per_block=npb if blocked else None, _internals=True)
+        x = d["x0"] * (1.0 + 0.03 * np.cos(np.arange(d["x0"].size)))
+        Zm = np.asarray(d["resid"](x)).reshape(n, n_pts) + z.astype(np.float64)
+    finally:
+        settings.use_x64(False)
+    ev = {k: np.asarray(getattr(events, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    nb = n_pts // npb if blocked else 1
+    Zo = np.zeros((n, n_pts))
+    for i in range(n):
+        for b in range(nb):
+            rec = _record(t64, i)
+            m_scale, sigma = None, 7.0
+            for p_, (l, members, blk) in enumerate(d["cols"]):
+                s = spec.slots[l]
+                if i not in members or (blk >= 0 and blk != b):
+                    continue
+                v = x[p_] * s.scale
+                if s.field == "m_start":
+                    m_scale = v
+                elif s.field == "b0_spread":
+                    sigma = v
+                else:
+                    cur = rec[s.field] if s.index is None else rec[s.field][s.index]
+                    new = v if s.mode == "value" else (cur * v if s.mode == "factor" else cur + v)
+                    if s.index is None:
+                        rec[s.field] = np.array(new)
+                    else:
+                        rec[s.field][s.index] = new
+            start = m0[i].astype(np.float64) if m_scale is None else O.initial_state(rec) * m_scale
+            rows = slice(b * npb, (b + 1) * npb) if blocked else slice(None)
+            evb = {k: v[rows] for k, v in ev.items()}
+            if spread is None:
+                nodes, weights = np.zeros(1), np.ones(1)
+            else:
+                nodes, weights = _spread_nodes(sp[0], sp[1])
+            hz = float(rec["gamma"]) * float(rec["b0"]) / (2 * np.pi)
+            acc = 0.0
+            for u, w in zip(nodes, weights):
+                q = dict(rec, b0_inhom=np.array(float(rec["b0_inhom"]) + sigma * u / hz))
+                mm, _ = OC.simulate(OC.pack(q)[None], len(q["cest_k"]), evb, start)
+                acc = acc + w * mm[0, :, p.mz_loc]
+            Zo[i, rows] = acc
+    assert np.max(np.abs(Zm - Zo)) < 1e-11, (case, np.max(np.abs(Zm - Zo)))

@@ -35,6 +35,33 @@ def test_forward_and_gradients_match_the_oracle(torch_mod):
     assert np.linalg.norm(g - go) < 5e-3 * np.linalg.norm(go)
 
 
+@pytest.mark.parametrize("batch", [4096, 1000])
+def test_the_proposal_network_at_its_size(torch_mod, batch):
+    """The proposal network itself, 244-512-512-256-5, at the training batch of 4096 (and a ragged 1000):
+    outputs and every layer's weight and bias gradients against the oracle."""
+    torch = torch_mod
+    from bmc_jax.learned import MLP
+    dims = (244, 512, 512, 256, 5)
+    mlp = MLP(dims=dims, max_batch=4096, seed=11)
+    rng = np.random.default_rng(2)
+    X = rng.uniform(0.0, 1.0, size=(batch, 244)).astype(np.float32)
+    T = rng.normal(size=(batch, 5)).astype(np.float32)
+    w = mlp.get_params().astype(np.float64)
+    Y = mlp.forward(torch.as_tensor(X, device="cuda")).cpu().numpy()
+    Yo, _, _ = MO.forward(w, dims, X)
+    assert np.max(np.abs(Y - Yo)) < 2e-3 * max(1.0, np.abs(Yo).max())
+    g, loss = mlp.gradients(torch.as_tensor(X, device="cuda"), torch.as_tensor(T, device="cuda"))
+    go, lo = MO.gradients(w, dims, X, T)
+    assert loss == pytest.approx(lo, rel=1e-3)
+    off = 0
+    for l in range(len(dims) - 1):
+        for n in (dims[l] * dims[l + 1], dims[l + 1]):
+            a, b = g[off:off + n], go[off:off + n]
+            assert np.linalg.norm(a - b) < 1e-2 * np.linalg.norm(b), (l, n, np.linalg.norm(a - b) / np.linalg.norm(b))
+            off += n
+    assert off == g.size == go.size
+
+
 def test_the_network_learns_a_smooth_map(torch_mod):
     torch = torch_mod
     from bmc_jax.learned import MLP

@@ -65,30 +65,26 @@ def problems():
 
 
 def _uniform_sim(p, x_scaled):
+    """The uniform path on one problem with its forward-mode Jacobian kernel, the arithmetic the uneven
+    sweep shares when asked for forward mode (single-event rows otherwise take the reverse kernel, equal
+    to rounding: tests/test_cw_reverse.py)."""
     from bmc_jax import _engine
     from bmc_jax.fitting.params import write_params
     from bmc_jax.fitting.residual import spec_slots
     nc = len(p.params.cest_k)
     pw = write_params(p.params, x_scaled, p.spec)
     m, J = _engine.simulate_with_jac(pw, p.events, p.m0, spec_slots(p.spec, nc), p.spec.scales, 2 * (1 + nc),
-                                     n_voxels=1, dtype=np.float64)
+                                     n_voxels=1, dtype=np.float64, forward_jacobian=True)
     return m[0, :, 2 * (1 + nc)], J[0]
 
 
-@pytest.fixture
-def forward_jacobian(monkeypatch):
-    """The uniform path with its forward-mode Jacobian kernel, the arithmetic the uneven sweep shares
-    (single-event rows otherwise take the reverse kernel, equal to rounding: tests/test_cw_reverse.py)."""
-    monkeypatch.setenv("BMC_FORWARD_JACOBIAN", "1")
-
-
 @pytest.mark.parametrize("cfg", [(1, 1024, 1), (2, 256, 0), (1, 7, 0)])
-def test_simulate_matches_the_uniform_kernel(problems, cfg, forward_jacobian):
+def test_simulate_matches_the_uniform_kernel(problems, cfg):
     from bmc_jax.fitting.uneven import UnevenBatch
     b = UnevenBatch(problems, dtype=np.float64)
     b.configure(*cfg)
     x = b.x0_scaled()
-    z, J = b.simulate(x)
+    z, J = b.simulate(x, forward_jacobian=True)
     z = z.cpu().numpy()
     for i, p in enumerate(problems):
         zu, Ju = _uniform_sim(p, x[b.x_off[i]:b.x_off[i + 1]])
@@ -97,19 +93,19 @@ def test_simulate_matches_the_uniform_kernel(problems, cfg, forward_jacobian):
         np.testing.assert_allclose(b.jacobian_of(J, i), Ju, rtol=1e-12, atol=1e-14 * np.abs(Ju).max())
 
 
-def test_lm_matches_the_uniform_lm(problems, forward_jacobian):
+def test_lm_matches_the_uniform_lm(problems):
     from bmc_jax.fitting.independent import _lm_batched
     from bmc_jax.fitting.uneven import UnevenBatch
     from bmc_jax.simulation.voxels import tile_params
     b = UnevenBatch(problems, dtype=np.float64)
     b.configure(2, 64, 1)
-    x, cost, nd = b.lm_fit(n_iter=6)
+    x, cost, nd = b.lm_fit(n_iter=6, forward_jacobian=True)
     xs = b.split(x)
     for i, p in enumerate(problems):
         nc = len(p.params.cest_k)
         x0 = p.spec.to_scaled(p.x0_phys)[None]
         xu, _, _, _ = _lm_batched(tile_params(p.params, 1), p.events, p.m0[None], 2 * (1 + nc), p.spec,
-                                  np.asarray(p.z_meas)[None], x0, 6, np.float64)
+                                  np.asarray(p.z_meas)[None], x0, 6, np.float64, forward_jacobian=True)
         np.testing.assert_allclose(xs[i], xu[0] * p.spec.scales, rtol=1e-11)
     assert np.all(nd == 6)
 
@@ -132,7 +128,7 @@ def test_graph_and_early_exit_and_recovery(problems):
     assert np.median(rms) <= 1e-5, rms
 
 
-def test_reverse_jacobian_sweep_matches_forward_mode(problems, monkeypatch):
+def test_reverse_jacobian_sweep_matches_forward_mode(problems):
     """The uneven batch's Jacobian sweep of single-event rows by reverse accumulation (one item per row,
     every tangent at once) against its forward-mode sweep (items per tangent pass)."""
     from bmc_jax.fitting.uneven import UnevenBatch
@@ -141,8 +137,7 @@ def test_reverse_jacobian_sweep_matches_forward_mode(problems, monkeypatch):
     x = b.x0_scaled()
     z_r, J_r = b.simulate(x)
     z_r = z_r.cpu().numpy()
-    monkeypatch.setenv("BMC_FORWARD_JACOBIAN", "1")
-    z_f, J_f = b.simulate(x)
+    z_f, J_f = b.simulate(x, forward_jacobian=True)
     # different Taylor degree / squaring splits of exp(X): equal to rounding, the oracle's 1e-11 bar
     np.testing.assert_allclose(z_r, z_f.cpu().numpy(), rtol=0, atol=1e-11)
     for i in range(len(problems)):

@@ -197,3 +197,123 @@ def test_golden_configuration_1_is_reproduced():
     m, dm = OC.simulate(OC.pack(p)[None], 1, ev, O.initial_state(p), slots=idx)
     np.testing.assert_allclose(m[0, :, 4], g["z"], rtol=0, atol=1e-14)
     np.testing.assert_allclose(dm[0, :, :, 4], g["jac"], rtol=1e-10, atol=0)
+
+
+# ---- pins added for the lineshapes, the MT block, the LM rules and the CW amplitude -----------------
+
+def _integral(f, edges):
+    from scipy.integrate import quad
+    return sum(quad(f, a, b, limit=400, epsabs=0.0, epsrel=1e-11)[0] for a, b in zip(edges[:-1], edges[1:]))
+
+
+def test_lineshape_integrals():
+    """Every node of the SuperLorentzian rule is a unit-area Gaussian in the offset, so the 101-node rule
+    with its pi * du weight integrates to exactly 1.01 pi; the Lorentzian t2 / (1 + (d t2)^2) carries no
+    1/pi and integrates to pi (matrix.py:26-38, 70-81). A dropped pi, du or sqrt(2/pi), or a lineshape
+    normalised to 1, fails by a factor."""
+    t2 = 1e-5
+    # node widths run from 0.0092 / (2 t2) to 2 / (2 t2) rad/s: split the half line where they change
+    edges = [0.0, 2e3, 1e4, 5e4, 2e5, 6e5, 3e6]
+    half = _integral(lambda d: O.interpolate_sl(d, t2), edges)
+    assert 2 * half == pytest.approx(1.01 * np.pi, rel=1e-8)
+    t2l = 0.01
+    from scipy.integrate import quad
+    half_l = quad(lambda d: O.lorentzian(d, t2l), 0.0, np.inf, epsabs=0.0, epsrel=1e-12)[0]
+    assert 2 * half_l == pytest.approx(np.pi, rel=1e-9)
+
+
+def test_mt_saturation_rate_is_henkelman():
+    """The MT pool's RF saturation rate is the textbook pi w1^2 g(delta) with g a unit-area lineshape
+    (Henkelman et al. 1993): for the Lorentzian, g = (T2 / pi) / (1 + (delta T2)^2). The diagonal of the
+    MT row must be -(R1 + k + pi w1^2 g), with w1 = 2 pi amp rel_b1 (matrix.py:121, 210, 282-286)."""
+    p = O.make_params(1.3, 0.07, pools=[dict(f=2e-3, k=100.0, dw=3.5, t1=1.0, t2=0.02)],
+                      mt=dict(f=0.12, k=25.0, dw=-2.5, t1=1.1, t2=1e-5, lineshape="lorentzian"), rel_b1=0.9)
+    amp, freq = 85.0, 1500.0
+    A, _ = O.bmc_matrix(p, amp, 0.3, freq)
+    larmor = p["gamma"] * 1e6 * p["b0"]
+    delta = 2 * np.pi * freq - p["mt_dw"] * 1e-6 * larmor          # MT offset: RF frequency past the MT line
+    w1 = 2 * np.pi * amp * p["rel_b1"]
+    t2 = 1.0 / p["mt_r2"]
+    g = (t2 / np.pi) / (1 + (delta * t2) ** 2)
+    zmt = 3 * (1 + 1)
+    assert A[zmt, zmt] == pytest.approx(-(p["mt_r1"] + p["mt_k"] + np.pi * w1 ** 2 * g), rel=1e-13)
+
+
+@pytest.mark.parametrize("lineshape", ["lorentzian", "superlorentzian"])
+def test_equilibrium_with_an_mt_pool_is_stationary(lineshape):
+    """With no RF, the start state (water 1, pools f, MT f_mt) is the equilibrium of the exchange: it must
+    not move over 10 s in either oracle. Fails if C[zmt] lacks f_mt, k_mt is not weighted by f_mt on the
+    water side, or a sign or index of the MT block is wrong (matrix.py:282-293)."""
+    p = O.make_params(1.3, 0.07, pools=[dict(f=3e-3, k=200.0, dw=3.5, t1=1.0, t2=0.02),
+                                        dict(f=6e-3, k=900.0, dw=2.0, t1=1.2, t2=0.03)],
+                      mt=dict(f=0.13, k=40.0, dw=-2.5, t1=0.9, t2=1e-5, lineshape=lineshape))
+    M0 = O.initial_state(p)
+    A, C = O.bmc_matrix(p, 0.0, 0.0, 250.0)
+    assert np.max(np.abs(A @ M0 + C)) < 1e-14
+    row = dict(type=np.array([0, 0]), dt=np.array([4.0, 6.0]), rf_amp=np.zeros(2), rf_phase=np.zeros(2),
+               rf_freq=np.array([0.0, 250.0]))
+    M = O.simulate_sequence(p, row, M0)
+    assert np.max(np.abs(M - M0)) < 1e-12
+    ev = {k: v[None] for k, v in row.items()}
+    m_c, _ = OC.simulate(OC.pack(p)[None], 2, ev, M0)
+    assert np.max(np.abs(m_c[0, 0] - M0)) < 1e-12
+    # the check bites: the MT source unweighted by its fraction, or the water-side rate unweighted, move it
+    zmt, zw = 3 * 3, 2 * 3
+    bad_c = C.copy()
+    bad_c[zmt] = p["mt_r1"]
+    bad_a = A.copy()
+    bad_a[zw, zw] += p["mt_k"] * p["mt_f"] - p["mt_k"]
+    assert np.max(np.abs(A @ M0 + bad_c)) > 1e-3 and np.max(np.abs(bad_a @ M0 + C)) > 1e-3
+
+
+def test_cw_nutation_closed_form():
+    """Relaxation and pools removed, water alone under CW saturation from Mz = 1 nutates about the
+    effective field: Mz(t) = (d^2 + w1^2 cos(W t)) / W^2 with W^2 = d^2 + w1^2, w1 = gamma B1 =
+    b1 * 267.5153 rad/s and d = 2 pi ppm 42.5764 B0 (saturation.py:79-83, matrix.py:121-162). The float32
+    storage of the event values (saturation.py:79-83) bounds the agreement to ~1e-6."""
+    p = O.make_params(np.inf, np.inf, pools=[], b0=3.0)
+    assert p["water_r1"] == 0.0 and p["water_r2"] == 0.0
+    b1, t = 1.3, 0.05
+    ppm = np.array([0.0, 0.002, -0.004])
+    ev = O.cw_saturation(ppm, b1_uT=b1, tsat_s=t, b0_T=3.0)
+    Mz = O.simulate_spectrum(p, ev, O.initial_state(p))[:, 2]
+    w1 = b1 * 267.5153
+    d = 2 * np.pi * ppm * 42.5764 * 3.0
+    W2 = d ** 2 + w1 ** 2
+    want = (d ** 2 + w1 ** 2 * np.cos(np.sqrt(W2) * t)) / W2
+    np.testing.assert_allclose(Mz, want, rtol=0, atol=5e-6)
+    # gamma in rad/s/uT and in Hz/T are the same constant
+    assert 267.5153 / (2 * np.pi) == pytest.approx(42.5764, rel=2e-6)
+
+
+def test_lm_trajectory_by_hand():
+    """lm.py:43-61 on r(x) = atan(x - 3), x0 = 0, lam0 = 1e-2, no bounds in play. By hand, with
+    J = 1/10, r = atan(-3): delta = -J r / (J^2 (1 + lam) + 1e-9 lam). The first five trials overshoot
+    past x = 6.9 and raise the cost (each rejected, lam x 3: 0.01, 0.03, 0.09, 0.27, 0.81); the sixth, at
+    lam = 2.43, lands at x = 3.6415326061487 and is accepted (lam x 0.5); then 3.27803114950289 and
+    3.09629255683953. A swapped factor, a missing diag(H) or a rejected-step update of x fails."""
+    r = lambda x: np.array([np.arctan(x[0] - 3.0)])
+    J = lambda x: np.array([[1.0 / (1.0 + (x[0] - 3.0) ** 2)]])
+    x, X, C = O.levenberg_marquardt(r, J, [0.0], [-100.0], [100.0], n_iter=8, trajectory=True)
+    np.testing.assert_array_equal(X[:5, 0], 0.0)
+    np.testing.assert_allclose(C[:5], np.arctan(3.0) ** 2, rtol=1e-15)
+    np.testing.assert_allclose(X[5:, 0], [3.6415326061487225, 3.2780311495028904, 3.0962925568395328], rtol=1e-13)
+
+
+def test_lm_damping_floor_and_strict_accept():
+    """The 1e-9 floor is scaled by lam, not added to diag(H) alone: with J = 1e-6 the first step of
+    r(x) = 1e-6 x - 1 from 0 is 1e-6 / (1e-12 * 1.01 + 1e-11) = 90826.5 (without the floor 990099, with an
+    unscaled floor 998.99). A clipped step that leaves x where it was ties the cost and is rejected
+    (strict <): lam grows x 3 every iteration, x never moves."""
+    r = lambda x: np.array([1e-6 * x[0] - 1.0])
+    J = lambda x: np.array([[1e-6]])
+    x, X, _ = O.levenberg_marquardt(r, J, [0.0], [-1e7], [1e7], n_iter=1, trajectory=True)
+    assert X[0, 0] == pytest.approx(90826.5213442325, rel=1e-12)
+    r2 = lambda x: np.array([x[0] - 3.0])
+    J2 = lambda x: np.array([[1.0]])
+    x, X, C = O.levenberg_marquardt(r2, J2, [0.0], [-1.0], [0.0], n_iter=4, trajectory=True)
+    np.testing.assert_array_equal(X[:, 0], 0.0)
+    np.testing.assert_array_equal(C, 9.0)
+    # lam after 4 ties is 0.01 * 3^4; the next free step from 0 towards 3 shows it
+    x5, X5, _ = O.levenberg_marquardt(r2, J2, [0.0], [-1.0], [10.0], n_iter=1, lam0=0.01 * 3 ** 4, trajectory=True)
+    assert X5[0, 0] == pytest.approx(3.0 / (1.0 + 0.81 * (1 + 1e-9)), rel=1e-14)
