@@ -78,6 +78,7 @@ def lib():
         L.bmc_lm_fit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ip32, dp, dp, dp,
                                  ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_uint,
                                  ctypes.c_double, ctypes.c_double, vp, vp, vp, vp]
+        L.bmc_lm_fit_noalloc.argtypes = L.bmc_lm_fit.argtypes
         L.bmc_tangents_per_pass.argtypes = [ctypes.c_int]
         L.bmc_tangents_per_pass.restype = ctypes.c_int
         L.bmc_adjoint_workspace_size.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, i64p]
@@ -85,7 +86,7 @@ def lib():
                                            dp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int64, vp]
         for name in ("bmc_plan_create", "bmc_plan_create_ex", "bmc_plan_destroy", "bmc_plan_stats", "bmc_plan_program",
                      "bmc_plan_compile",
-                     "bmc_simulate", "bmc_simulate_ex", "bmc_simulate_trajectory", "bmc_lm_fit", "bmc_adjoint_workspace_size",
+                     "bmc_simulate", "bmc_simulate_ex", "bmc_simulate_trajectory", "bmc_lm_fit", "bmc_lm_fit_noalloc", "bmc_adjoint_workspace_size",
                      "bmc_simulate_adjoint"):
             getattr(L, name).restype = ctypes.c_int
         if L.bmc_abi_version() != ABI_VERSION:
@@ -250,9 +251,12 @@ def simulate_trajectory(plan: Plan, params, n_cest: int, dtype_code: int, m0):
 
 
 def lm_fit(plan: Plan, params, n_cest: int, dtype_code: int, m0, slots, scales, lo, hi, jc, z_meas, x0, n_iter,
-           lam0=1e-2, free=None, flags=0, early_ftol=0.0, early_rms=0.0, want_traj=False, want_cost=False):
+           lam0=1e-2, free=None, flags=0, early_ftol=0.0, early_rms=0.0, want_traj=False, want_cost=False,
+           noalloc=False):
     """bmc_lm_fit on device tensors. x0 [V, P] float64 (scaled). Returns (x, cost_traj, x_traj, n_done);
-    cost_traj [V, n_iter] comes with want_traj or want_cost, x_traj only with want_traj."""
+    cost_traj [V, n_iter] comes with want_traj or want_cost, x_traj only with want_traj. noalloc: the
+    allocation-free entry point bmc_lm_fit_noalloc (its workspace is the library's fixed device array;
+    it does not synchronise the stream)."""
     torch = torch_cuda()
     V, P = int(x0.shape[0]), len(slots)
     x = x0.to(device="cuda", dtype=torch.float64).contiguous().clone()
@@ -262,11 +266,13 @@ def lm_fit(plan: Plan, params, n_cest: int, dtype_code: int, m0, slots, scales, 
     arr = lambda ct, vals: (ct * max(len(vals), 1))(*vals)
     ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
     fr = None if free is None else free.to(device="cuda", dtype=torch.float64).contiguous()
-    check(lib().bmc_lm_fit(plan.handle, V, n_cest, dtype_code, ptr(params), ptr(m0), P,
+    entry = lib().bmc_lm_fit_noalloc if noalloc else lib().bmc_lm_fit
+    check(entry(plan.handle, V, n_cest, dtype_code, ptr(params), ptr(m0), P,
                            arr(ctypes.c_int32, [int(s) for s in slots]), arr(ctypes.c_double, [float(s) for s in scales]),
                            arr(ctypes.c_double, [float(v) for v in lo]), arr(ctypes.c_double, [float(v) for v in hi]),
                            jc, ptr(z_meas), ptr(x), int(n_iter), float(lam0), ptr(fr), int(flags), float(early_ftol),
-                           float(early_rms), ptr(ctraj), ptr(xtraj), ptr(ndone), stream_ptr()), "bmc_lm_fit")
+                           float(early_rms), ptr(ctraj), ptr(xtraj), ptr(ndone), stream_ptr()),
+          "bmc_lm_fit_noalloc" if noalloc else "bmc_lm_fit")
     return x, ctraj, xtraj, ndone
 
 

@@ -13,6 +13,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -250,72 +251,62 @@ __global__ void init_kernel(LmState s, int V, double lam0) {
   }
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
+// The work buffers of one fit, carved from one block of device memory (each part 256-byte aligned).
+struct LmWork {
+  double *z, *J, *zt, *xt, *lam, *cost;
+  int32_t *active, *jac_live, *it, *list_j, *list_a, *cnt;
 };
 
-}  // namespace
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-}  // namespace bmc
+size_t lm_work_bytes(int V, int O, int P) {
+  const size_t v = (size_t)V, o = (size_t)O, p = (size_t)P;
+  return align256(8 * v * o) + align256(8 * v * o * p) + align256(8 * v * o) + align256(8 * v * p) +
+         2 * align256(8 * v) + 4 * align256(4 * v) + align256(4) + align256(8);
+}
 
-using namespace bmc;
+void lm_carve(char* base, int V, int O, int P, LmWork& w) {
+  const size_t v = (size_t)V, o = (size_t)O, p = (size_t)P;
+  auto take = [&](size_t n) {
+    char* q = base;
+    base += align256(n);
+    return q;
+  };
+  w.z = (double*)take(8 * v * o);
+  w.J = (double*)take(8 * v * o * p);
+  w.zt = (double*)take(8 * v * o);
+  w.xt = (double*)take(8 * v * p);
+  w.lam = (double*)take(8 * v);
+  w.cost = (double*)take(8 * v);
+  w.active = (int32_t*)take(4 * v);
+  w.jac_live = (int32_t*)take(4 * v);
+  w.list_j = (int32_t*)take(4 * v);
+  w.list_a = (int32_t*)take(4 * v);
+  w.it = (int32_t*)take(4);
+  w.cnt = (int32_t*)take(8);
+}
 
-extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
-                          const void* m0_dev, int n_params, const int32_t* slot, const double* scale, const double* lo,
-                          const double* hi, int jac_component, const void* z_meas_dev, double* x_dev, int n_iter,
-                          double lam0, const double* free_dev, unsigned flags, double early_ftol, double early_rms,
-                          double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev, void* stream_) {
-  const int V = n_voxels, P = n_params;
-  if (P <= 0 || P > kMaxP || !lo || !hi || !x_dev || !z_meas_dev || n_iter < 0) {
-    set_error("bmc_lm_fit: need 1.." + std::to_string(kMaxP) + " free values, bounds, x and z_meas");
-    return BMC_EINVAL;
-  }
+// One fit on a carved workspace: every launch on `stream`, in stream order, no allocation and no
+// synchronisation (the caller owns the workspace's lifetime). Every buffer a kernel reads is written
+// earlier in the same call (init_kernel, clear2_kernel, the sweeps), so the workspace's contents on
+// entry do not matter.
+int lm_run(const bmc_plan* plan, int V, int n_cest, int dtype, const void* params_dev, const void* m0_dev, int P,
+           const int32_t* slot, const double* scale, const double* lo, const double* hi, int jac_component,
+           const void* z_meas_dev, double* x_dev, int n_iter, double lam0, const double* free_dev, unsigned flags,
+           double early_ftol, double early_rms, double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev,
+           const LmWork& w, hipStream_t stream) {
   SimArgs sj;
   int rc = fill_sim_args(plan, V, n_cest, dtype, params_dev, m0_dev, nullptr, P, slot, scale, jac_component, sj);
   if (rc != BMC_OK) return rc;
-  if (V == 0 || n_iter == 0) return BMC_OK;
   const int O = plan->p.O;
-  hipStream_t stream = (hipStream_t)stream_;
-  const bool use_graph = (flags & BMC_LM_GRAPH) != 0;
-  hipStream_t own = nullptr;
-  if (use_graph && stream == nullptr) {
-    BMC_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
-    stream = own;
-  }
-  struct StreamGuard {
-    hipStream_t s;
-    ~StreamGuard() {
-      if (s) (void)hipStreamDestroy(s);
-    }
-  } sg{own};
-
-  DevBuf bz, bJ, bzt, bxt, blam, bcost, bact, bjl, bit, blj, bla, bcnt;
-  auto alloc = [&](DevBuf& b, size_t n) -> int {
-    if (hipMalloc(&b.p, n > 0 ? n : 8) != hipSuccess) {
-      set_error("bmc_lm_fit: device allocation of " + std::to_string(n) + " bytes failed");
-      return BMC_ENOMEM;
-    }
-    return BMC_OK;
-  };
-  if ((rc = alloc(bz, sizeof(double) * V * O)) || (rc = alloc(bJ, sizeof(double) * (size_t)V * O * P)) ||
-      (rc = alloc(bzt, sizeof(double) * V * O)) || (rc = alloc(bxt, sizeof(double) * V * P)) ||
-      (rc = alloc(blam, sizeof(double) * V)) || (rc = alloc(bcost, sizeof(double) * V)) ||
-      (rc = alloc(bact, sizeof(int32_t) * V)) || (rc = alloc(bjl, sizeof(int32_t) * V)) ||
-      (rc = alloc(bit, sizeof(int32_t))) || (rc = alloc(blj, sizeof(int32_t) * V)) ||
-      (rc = alloc(bla, sizeof(int32_t) * V)) || (rc = alloc(bcnt, sizeof(int32_t) * 2)))
-    return rc;
-
   LmState st;
   st.x = x_dev;
-  st.xt = (double*)bxt.p;
-  st.lam = (double*)blam.p;
-  st.cost = (double*)bcost.p;
-  st.active = (int32_t*)bact.p;
-  st.jac_live = (int32_t*)bjl.p;
-  st.it = (int32_t*)bit.p;
+  st.xt = w.xt;
+  st.lam = w.lam;
+  st.cost = w.cost;
+  st.active = w.active;
+  st.jac_live = w.jac_live;
+  st.it = w.it;
   st.n_done = n_iter_done_dev;
   st.ctraj = cost_traj_dev;
   st.xtraj = x_traj_dev;
@@ -328,22 +319,22 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
     sj.w_slot[p] = slot[p];
     sj.w_scale[p] = scale[p];
   }
-  sj.z_out = (double*)bz.p;
-  sj.jac_out = bJ.p;
+  sj.z_out = w.z;
+  sj.jac_out = w.J;
   sj.jac_dtype = BMC_F64;
   // each sweep runs over a compacted list of the voxels it has to simulate
-  int32_t* cnt = (int32_t*)bcnt.p;
+  int32_t* cnt = w.cnt;
   sj.active = nullptr;
-  sj.vlist = (const int32_t*)blj.p;
+  sj.vlist = w.list_j;
   sj.n_vlist = cnt;
   sj.PT = tangents_per_pass(P);
   sj.n_pass = (P + sj.PT - 1) / sj.PT;
   // launch 3: primal at the trial point
   SimArgs sp = sj;
-  sp.vlist = (const int32_t*)bla.p;
+  sp.vlist = w.list_a;
   sp.n_vlist = cnt + 1;
   sp.xw = st.xt;
-  sp.z_out = (double*)bzt.p;
+  sp.z_out = w.zt;
   sp.jac_out = nullptr;
   sp.PT = 0;
   sp.n_pass = 1;
@@ -355,8 +346,8 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   ne.P = P;
   ne.dtype = dtype;
   ne.has_free = free_dev != nullptr;
-  ne.z = (const double*)bz.p;
-  ne.J = (const double*)bJ.p;
+  ne.z = w.z;
+  ne.J = w.J;
   ne.zmeas = z_meas_dev;
   ne.free_ = free_dev;
   for (int p = 0; p < P; p++) {
@@ -374,7 +365,7 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   ac.early = (flags & BMC_LM_EARLY_EXIT) ? 1 : 0;
   ac.ftol = early_ftol;
   ac.rms_tol = early_rms;
-  ac.zt = (const double*)bzt.p;
+  ac.zt = w.zt;
   ac.zmeas = z_meas_dev;
   ac.s = st;
 
@@ -387,7 +378,7 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
     s0.xw = x_dev;
     hipLaunchKernelGGL(clear2_kernel, dim3(1), dim3(64), 0, stream, cnt);
     hipLaunchKernelGGL(compact_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, (const int32_t*)st.active, V,
-                       (int32_t*)bla.p, cnt + 1);
+                       w.list_a, cnt + 1);
     BMC_HIP_CHECK(hipGetLastError());
     if ((rc = launch_sim(n_cest, s0, stream)) != BMC_OK) return rc;
     hipLaunchKernelGGL(start_kernel, dim3(V), dim3(64), 0, stream, ac);
@@ -398,14 +389,14 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
     int r;
     hipLaunchKernelGGL(clear2_kernel, dim3(1), dim3(64), 0, stream, cnt);
     hipLaunchKernelGGL(compact_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, (const int32_t*)st.jac_live, V,
-                       (int32_t*)blj.p, cnt);
+                       w.list_j, cnt);
     BMC_HIP_CHECK(hipGetLastError());
     if ((r = cwj ? launch_cwj(n_cest, sj, plan->p.d_live, stream) : launch_sim(n_cest, sj, stream)) != BMC_OK)
       return r;
     hipLaunchKernelGGL(normal_eq_kernel, dim3(V), dim3(64), 0, stream, ne);
     BMC_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(compact_kernel, dim3((V + 255) / 256), dim3(256), 0, stream, (const int32_t*)st.active, V,
-                       (int32_t*)bla.p, cnt + 1);
+                       w.list_a, cnt + 1);
     BMC_HIP_CHECK(hipGetLastError());
     if ((r = launch_sim(n_cest, sp, stream)) != BMC_OK) return r;
     hipLaunchKernelGGL(accept_kernel, dim3(V), dim3(64), 0, stream, ac);
@@ -415,39 +406,172 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
     return BMC_OK;
   };
 
-  if (!use_graph) {
+  if (!(flags & BMC_LM_GRAPH)) {
     for (int it = 0; it < n_iter; it++)
       if ((rc = iteration()) != BMC_OK) return rc;
-  } else {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    BMC_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    rc = iteration();
-    hipError_t ce = hipStreamEndCapture(stream, &graph);
-    if (rc != BMC_OK) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc;
-    }
-    if (ce != hipSuccess) {
-      set_error(std::string("bmc_lm_fit: graph capture: ") + hipGetErrorString(ce));
-      return BMC_EHIP;
-    }
-    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      set_error(std::string("bmc_lm_fit: graph instantiate: ") + hipGetErrorString(ie));
-      return BMC_EHIP;
-    }
-    hipError_t le = hipSuccess;
-    for (int it = 0; it < n_iter && le == hipSuccess; it++) le = hipGraphLaunch(exec, stream);
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    if (le != hipSuccess) {
-      set_error(std::string("bmc_lm_fit: graph launch: ") + hipGetErrorString(le));
-      return BMC_EHIP;
-    }
+    return BMC_OK;
   }
-  // buffers are freed on return: finish the work that uses them first
-  BMC_HIP_CHECK(hipStreamSynchronize(stream));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  BMC_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  rc = iteration();
+  hipError_t ce = hipStreamEndCapture(stream, &graph);
+  if (rc != BMC_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ce != hipSuccess) {
+    set_error(std::string("bmc_lm_fit: graph capture: ") + hipGetErrorString(ce));
+    return BMC_EHIP;
+  }
+  hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    set_error(std::string("bmc_lm_fit: graph instantiate: ") + hipGetErrorString(ie));
+    return BMC_EHIP;
+  }
+  hipError_t le = hipSuccess;
+  for (int it = 0; it < n_iter && le == hipSuccess; it++) le = hipGraphLaunch(exec, stream);
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (le != hipSuccess) {
+    set_error(std::string("bmc_lm_fit: graph launch: ") + hipGetErrorString(le));
+    return BMC_EHIP;
+  }
   return BMC_OK;
+}
+
+int lm_check(int P, const double* lo, const double* hi, const double* x_dev, const void* z_meas_dev, int n_iter,
+             const char* who) {
+  if (P <= 0 || P > kMaxP || !lo || !hi || !x_dev || !z_meas_dev || n_iter < 0) {
+    set_error(std::string(who) + ": need 1.." + std::to_string(kMaxP) + " free values, bounds, x and z_meas");
+    return BMC_EINVAL;
+  }
+  return BMC_OK;
+}
+
+}  // namespace
+
+}  // namespace bmc
+
+using namespace bmc;
+
+extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                          const void* m0_dev, int n_params, const int32_t* slot, const double* scale, const double* lo,
+                          const double* hi, int jac_component, const void* z_meas_dev, double* x_dev, int n_iter,
+                          double lam0, const double* free_dev, unsigned flags, double early_ftol, double early_rms,
+                          double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev, void* stream_) {
+  const int V = n_voxels, P = n_params;
+  int rc = lm_check(P, lo, hi, x_dev, z_meas_dev, n_iter, "bmc_lm_fit");
+  if (rc != BMC_OK) return rc;
+  SimArgs probe;  // argument checks before anything is allocated
+  if ((rc = fill_sim_args(plan, V, n_cest, dtype, params_dev, m0_dev, nullptr, P, slot, scale, jac_component,
+                          probe)) != BMC_OK)
+    return rc;
+  if (V == 0 || n_iter == 0) return BMC_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  hipStream_t own = nullptr;
+  if ((flags & BMC_LM_GRAPH) && stream == nullptr) {  // a graph is captured on a stream of its own
+    BMC_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    stream = own;
+  }
+  struct Guard {
+    hipStream_t s = nullptr;
+    void* mem = nullptr;
+    ~Guard() {
+      if (mem) (void)hipFree(mem);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } guard;
+  guard.s = own;
+  const size_t bytes = lm_work_bytes(V, plan->p.O, P);
+  if (hipMalloc(&guard.mem, bytes) != hipSuccess) {
+    guard.mem = nullptr;
+    set_error("bmc_lm_fit: device allocation of " + std::to_string(bytes) + " bytes failed");
+    return BMC_ENOMEM;
+  }
+  LmWork w;
+  lm_carve((char*)guard.mem, V, plan->p.O, P, w);
+  rc = lm_run(plan, V, n_cest, dtype, params_dev, m0_dev, P, slot, scale, lo, hi, jac_component, z_meas_dev, x_dev,
+              n_iter, lam0, free_dev, flags, early_ftol, early_rms, cost_traj_dev, x_traj_dev, n_iter_done_dev, w,
+              stream);
+  // the workspace is freed on return: finish the work that uses it first
+  hipError_t se = hipStreamSynchronize(stream);
+  if (rc != BMC_OK) return rc;
+  if (se != hipSuccess) {
+    set_error(std::string("bmc_lm_fit: hipStreamSynchronize: ") + hipGetErrorString(se));
+    return BMC_EHIP;
+  }
+  return BMC_OK;
+}
+
+// ---- the allocation-free variant -----------------------------------------------------------------
+// Its workspace is this file-scope device array. Each device holds its own copy from the moment the
+// library's code object is loaded on it until the process ends. Host code never passes `scratch`
+// itself (on the host that name is only a stand-in for the symbol): it takes the device address from
+// hipGetSymbolAddress once per device and passes that.
+static __device__ __align__(256) unsigned char scratch[BMC_LM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+int scratch_of_current_device(void** out) {
+  int dev = -1;
+  BMC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("bmc_lm_fit_noalloc: device index " + std::to_string(dev) + " beyond the library's table");
+    return BMC_ENOTSUP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("bmc_lm_fit_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return BMC_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return BMC_OK;
+}
+
+}  // namespace
+
+extern "C" int bmc_lm_fit_noalloc(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                                  const void* m0_dev, int n_params, const int32_t* slot, const double* scale,
+                                  const double* lo, const double* hi, int jac_component, const void* z_meas_dev,
+                                  double* x_dev, int n_iter, double lam0, const double* free_dev, unsigned flags,
+                                  double early_ftol, double early_rms, double* cost_traj_dev, double* x_traj_dev,
+                                  int32_t* n_iter_done_dev, void* stream) {
+  const int V = n_voxels, P = n_params;
+  int rc = lm_check(P, lo, hi, x_dev, z_meas_dev, n_iter, "bmc_lm_fit_noalloc");
+  if (rc != BMC_OK) return rc;
+  if (flags & BMC_LM_GRAPH) {
+    set_error("bmc_lm_fit_noalloc: BMC_LM_GRAPH is refused (instantiating a graph allocates); run without it");
+    return BMC_EINVAL;
+  }
+  SimArgs probe;
+  if ((rc = fill_sim_args(plan, V, n_cest, dtype, params_dev, m0_dev, nullptr, P, slot, scale, jac_component,
+                          probe)) != BMC_OK)
+    return rc;
+  const size_t bytes = lm_work_bytes(V, plan->p.O, P);
+  if (bytes > (size_t)BMC_LM_NOALLOC_SCRATCH_BYTES) {
+    set_error("bmc_lm_fit_noalloc: " + std::to_string(V) + " voxels x " + std::to_string(plan->p.O) + " offsets x " +
+              std::to_string(P) + " values need " + std::to_string(bytes) + " bytes of workspace, above the fixed " +
+              std::to_string((size_t)BMC_LM_NOALLOC_SCRATCH_BYTES) + "; split the voxels or call bmc_lm_fit");
+    return BMC_EINVAL;
+  }
+  if (V == 0 || n_iter == 0) return BMC_OK;
+  void* base = nullptr;
+  if ((rc = scratch_of_current_device(&base)) != BMC_OK) return rc;
+  LmWork w;
+  lm_carve((char*)base, V, plan->p.O, P, w);
+  return lm_run(plan, V, n_cest, dtype, params_dev, m0_dev, P, slot, scale, lo, hi, jac_component, z_meas_dev, x_dev,
+                n_iter, lam0, free_dev, flags, early_ftol, early_rms, cost_traj_dev, x_traj_dev, n_iter_done_dev, w,
+                (hipStream_t)stream);
 }

@@ -181,6 +181,24 @@ int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const 
                double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev, void* stream);
 
 /*
+ * bmc_lm_fit without any device allocation. No counterpart in the reference: it is for callers that
+ * cannot call hipMalloc (work issued under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). Arguments, arithmetic and results are bmc_lm_fit's (the same kernels); the work buffers
+ * come from a fixed device array of BMC_LM_NOALLOC_SCRATCH_BYTES that the library holds on every device
+ * it runs on until the process ends. A call needs about 8 V (O (P + 2) + P + 2) + 16 V bytes (each of
+ * its twelve buffers rounded up to 256 B); a larger call returns BMC_EINVAL with the size it needed.
+ * BMC_LM_GRAPH is refused (instantiating a graph allocates). The call does not synchronise `stream`.
+ * Every call on one device shares the one array: calls issued in order on one stream are ordered, but
+ * calls that overlap on one GPU (other streams, other threads) race and corrupt each other.
+ */
+#define BMC_LM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int bmc_lm_fit_noalloc(const bmc_plan* plan, int n_voxels, int n_cest, int dtype, const void* params_dev,
+                       const void* m0_dev, int n_params, const int32_t* slot, const double* scale, const double* lo,
+                       const double* hi, int jac_component, const void* z_meas_dev, double* x_dev, int n_iter,
+                       double lam0, const double* free_dev, unsigned flags, double early_ftol, double early_rms,
+                       double* cost_traj_dev, double* x_traj_dev, int32_t* n_iter_done_dev, void* stream);
+
+/*
  * Reverse mode of bmc_simulate with respect to the events' RF amplitudes and RF frequencies: given
  * cotangents on Z = state[jac_component] and on the Jacobian J[v, o, p] that bmc_simulate returns with
  * the same tangents, adds dL/d rf_amp[o, t] and dL/d rf_freq[o, t], summed over voxels, into the
