@@ -11,28 +11,35 @@ from ..simulation.params import n_cest_of
 from ..simulation.simulate import simulate_spectrum
 
 
-def _check(experiment, spec):
-    if np.ndim(experiment.params.water_r1) != 0:
-        raise ValueError("the bound is computed for one sample: give an Experiment whose params carry one sample")
+def _require_one_plain_sample(experiment, spec):
+    """The bound is defined here for a single sample whose values are plain per-sample values
+    (crlb.py:30-35)."""
+    n_dims = np.ndim(experiment.params.water_r1)
+    if n_dims:
+        raise ValueError(f"crlb needs the parameters of one sample (scalar fields); these have {n_dims} batch "
+                         "axis")
     if not spec.independent:
-        raise ValueError("the bound supports plain values per sample (FreeParam scope='sample', mode='value', no "
-                         "samples=, no m_start or b0_spread)")
+        raise ValueError("crlb takes plain values per sample only: no shared or per-block scope, no samples=, "
+                         "no factor or offset mode, no m_start or b0_spread")
 
 
 def model_of(experiment, spec):
-    """x (physical) -> Z (crlb.py:38-47); carries .jac(x) = dZ/dx from the GPU."""
-    _check(experiment, spec)
+    """The model x (physical) -> Z at the experiment's offsets (crlb.py:38-47); the returned function
+    carries .jac(x) = dZ/dx from the GPU kernel's tangents."""
+    _require_one_plain_sample(experiment, spec)
     base, events, m0, mz = experiment.params, experiment.events, experiment.m0, experiment.mz_loc
     scales = spec.scales
     slots = spec_slots(spec, n_cest_of(base))
 
+    def placed(x):
+        return write_params(base, np.asarray(x, np.float64) / scales, spec)
+
     def z(x):
-        p = write_params(base, np.asarray(x, np.float64) / scales, spec)
-        return simulate_spectrum(p, events, m0, return_full_trajectory=False)[:, mz]
+        return simulate_spectrum(placed(x), events, m0, return_full_trajectory=False)[:, mz]
 
     def jac(x):
-        p = write_params(base, np.asarray(x, np.float64) / scales, spec)
-        _, J = _engine.simulate_with_jac(p, events, m0, slots, [1.0] * len(slots), mz, n_voxels=1, want_m=False)
+        _, J = _engine.simulate_with_jac(placed(x), events, m0, slots, np.ones(len(slots)), mz, n_voxels=1,
+                                         want_m=False)
         return J[0]
 
     z.jac = jac
@@ -40,13 +47,16 @@ def model_of(experiment, spec):
 
 
 def fisher_information(experiment, spec, x, sigma):
-    """F = J^T J / sigma^2 at x (physical) and J (crlb.py:50-58)."""
+    """(F, J): F = J^T J / sigma^2 at x (physical units), J = dZ/dx (crlb.py:50-58)."""
     J = np.asarray(model_of(experiment, spec).jac(np.asarray(x, np.float64)), np.float64)
-    return J.T @ J / sigma ** 2, J
+    return (J.T @ J) / float(sigma) ** 2, J
 
 
 @dataclass(frozen=True)
 class CRLB:
+    """The bound at the values x (physical units): sd is the smallest standard deviation an unbiased
+    estimate of each value can have; corr the correlations of those estimates; fisher and jacobian the
+    matrices it came from (crlb.py:61-85)."""
     names: tuple
     x: np.ndarray
     sd: np.ndarray
@@ -56,24 +66,29 @@ class CRLB:
 
     @property
     def rel_sd(self):
+        """sd relative to the value, sd / |x|."""
         return self.sd / np.abs(self.x)
 
     def __getitem__(self, name):
-        return float(self.sd[self.names.index(name)])
+        """The bound of one value, by its FreeParam name."""
+        return float(self.sd[list(self.names).index(name)])
 
     def summary(self):
-        rows = [f"{'parameter':>10} {'value':>12} {'CRLB (SD)':>12} {'relative':>9}"]
-        for n, x, s, r in zip(self.names, self.x, self.sd, self.rel_sd):
-            rows.append(f"{n:>10} {x:>12.6g} {s:>12.4g} {100 * r:>8.2f}%")
-        return "\n".join(rows)
+        """One line per value: its name, the value, the bound and the bound in percent of the value."""
+        lines = ["parameter".rjust(10) + " " + "value".rjust(12) + " " + "CRLB (SD)".rjust(12) + " "
+                 + "relative".rjust(9)]
+        for name, value, bound, rel in zip(self.names, self.x, self.sd, self.rel_sd):
+            lines.append(f"{name:>10} {value:>12.6g} {bound:>12.4g} {100.0 * rel:>8.2f}%")
+        return "\n".join(lines)
 
 
 def crlb(experiment, spec, sigma, x=None):
-    """The bound of spec's values for one sample (crlb.py:88-120)."""
-    x = np.asarray(spec.x0_phys if x is None else x, float)
-    F, J = fisher_information(experiment, spec, x, sigma)
+    """The Cramer-Rao bound of spec's values for one sample at x (default: each value's x0), with noise
+    sigma on every point (crlb.py:88-120)."""
+    at = np.asarray(spec.x0_phys if x is None else x, float)
+    F, J = fisher_information(experiment, spec, at, sigma)
     F = np.asarray(F, np.float64)
     cov = np.linalg.inv(F)
-    sd = np.sqrt(np.clip(np.diag(cov), 0.0, None))
-    corr = cov / np.outer(sd, sd)
-    return CRLB(tuple(s.name for s in spec.slots), x, sd, corr, F, np.asarray(J))
+    sd = np.sqrt(np.maximum(np.diag(cov), 0.0))
+    corr = cov / sd[:, None] / sd[None, :]
+    return CRLB(tuple(s.name for s in spec.slots), at, sd, corr, F, np.asarray(J))

@@ -58,7 +58,8 @@ def optimize_design(build: Callable, spec, d0, lo, hi, sigma: float, truths, tar
     truths = np.atleast_2d(np.asarray(truths, float))
     d0, lo, hi = (np.asarray(a, float) for a in (d0, lo, hi))
     if not (d0.shape == lo.shape == hi.shape) or np.any(d0 <= lo) or np.any(d0 >= hi):
-        raise ValueError("d0, lo and hi need the same shape, with lo < d0 < hi")
+        raise ValueError(f"the start must lie strictly inside its box, lo < d0 < hi elementwise, with d0, lo and "
+                         f"hi of one shape (got {d0.shape}, {lo.shape}, {hi.shape})")
 
     def cost_of(d):
         exp = _numpy_experiment(build(d))
@@ -111,7 +112,8 @@ def optimize_design(build: Callable, spec, d0, lo, hi, sigma: float, truths, tar
     for step in range(n_steps):
         c, g = value_and_grad(u)
         if not (np.isfinite(c) and np.all(np.isfinite(g))):
-            raise FloatingPointError(f"cost or gradient not finite at step {step}; try float64 or a smaller lr")
+            raise FloatingPointError(f"step {step}: the design cost or its gradient is NaN or infinite (a smaller "
+                                     "lr or float64 may help)")
         history.append(float(c))
         m = b1 * m + (1 - b1) * g
         v = b2 * v + (1 - b2) * g ** 2

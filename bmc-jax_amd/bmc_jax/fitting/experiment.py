@@ -41,12 +41,12 @@ class Experiment:
     def stack(cls, experiments: Sequence["Experiment"]) -> "Experiment":
         """Join sequences along the offset axis, front-padding shorter ones (experiment.py:101-141)."""
         experiments = list(experiments)
-        if not experiments:
-            raise ValueError("stack needs at least one experiment")
-        locs = {p.mz_loc for p in experiments}
-        if len(locs) > 1:
-            raise ValueError(f"experiments disagree on mz_loc: {sorted(locs)}; they must describe "
-                             "the same pools in the same order")
+        if len(experiments) == 0:
+            raise ValueError("Experiment.stack was given no experiments; it needs at least one")
+        water_z = sorted({p.mz_loc for p in experiments})
+        if len(water_z) != 1:
+            raise ValueError(f"the experiments read water Mz at different mz_loc {water_z}: stacked rungs must "
+                             "share one pool model")
         width = max(p.n_events for p in experiments)
         events = concat_events([pad_events(p.events, width) for p in experiments])
         first = experiments[0]

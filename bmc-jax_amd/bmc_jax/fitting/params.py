@@ -22,18 +22,26 @@ class FreeParam:
     mode: str = "value"
 
     def __post_init__(self):
-        if self.scope not in ("sample", "shared", "block"):
-            raise ValueError(f"{self.name}: scope {self.scope!r}; use 'sample', 'shared' or 'block'")
-        if self.mode not in ("value", "factor", "add"):
-            raise ValueError(f"{self.name}: mode {self.mode!r}; use 'value', 'factor' or 'add'")
-        if self.field == "m_start" and (self.index is not None or self.mode == "add"):
-            raise ValueError(f"{self.name}: m_start is one number per sample (index None), as a value or a factor")
-        if self.field == "b0_spread" and (self.index is not None or self.mode != "value"):
-            raise ValueError(f"{self.name}: b0_spread is one width in Hz (index None, mode 'value')")
+        """Checks the combination of scope, mode and field (params.py:52-63); normalises `samples` to a
+        sorted tuple of distinct ints."""
+        allowed = {"scope": ("sample", "shared", "block"), "mode": ("value", "factor", "add")}
+        for attr, choices in allowed.items():
+            if getattr(self, attr) not in choices:
+                raise ValueError(f"{self.name}: {attr} must be one of {choices}, not {getattr(self, attr)!r}")
+        if self.field == "m_start":
+            if self.index is not None or self.mode == "add":
+                raise ValueError(f"{self.name}: m_start is a single number per sample (index None), fitted as a "
+                                 "value or a factor, never as an offset")
+        elif self.field == "b0_spread":
+            if self.index is not None or self.mode != "value":
+                raise ValueError(f"{self.name}: b0_spread is the spread width itself in Hz (index None, "
+                                 "mode 'value')")
         if self.samples is not None:
-            object.__setattr__(self, "samples", tuple(sorted({int(i) for i in self.samples})))
-            if not self.samples:
-                raise ValueError(f"{self.name}: samples=() frees it nowhere; leave it out of the spec")
+            chosen = tuple(sorted(set(int(i) for i in self.samples)))
+            if len(chosen) == 0:
+                raise ValueError(f"{self.name}: an empty samples tuple frees the value in no sample; drop the "
+                                 "slot instead")
+            object.__setattr__(self, "samples", chosen)
 
 
 @dataclass(frozen=True)
@@ -67,19 +75,24 @@ class FitSpec:
     def to_phys(self, x_scaled):
         return np.asarray(x_scaled) * np.array([s.scale for s in self.slots])
 
+    def _needs_joint(self, s) -> bool:
+        """A slot that fitting every sample on its own cannot express."""
+        return (s.scope != "sample" or s.samples is not None or s.mode != "value"
+                or s.field in ("m_start", "b0_spread"))
+
     @property
     def independent(self) -> bool:
-        """True when every value is a plain per-sample value, free everywhere (what `fit` solves)."""
-        return all(s.scope == "sample" and s.samples is None and s.mode == "value"
-                   and s.field not in ("m_start", "b0_spread") for s in self.slots)
+        """True when every value is a plain per-sample value, free everywhere (what `fit` solves,
+        params.py:80-86)."""
+        return not any(self._needs_joint(s) for s in self.slots)
 
     def require_independent(self, caller: str) -> None:
-        if not self.independent:
-            joint = [s.name for s in self.slots if s.scope != "sample" or s.samples is not None
-                     or s.mode != "value" or s.field in ("m_start", "b0_spread")]
-            raise ValueError(f"{caller}: {joint} are shared, per block, free in some samples only, a factor / an "
-                             "offset, m_start or b0_spread (FreeParam scope / samples / mode / field), which "
-                             "samples fitted alone with plain values cannot do; use fit_joint")
+        """ValueError naming the slots that need fit_joint (params.py:88-97)."""
+        joint = [s.name for s in self.slots if self._needs_joint(s)]
+        if joint:
+            raise ValueError(f"{caller} fits every sample on its own with plain values; {joint} couple samples "
+                             "(shared or per-block scope), are free in some samples only, scale or shift a "
+                             "carried value, or are m_start / b0_spread: fit them with fit_joint")
 
 
 def write_params(base: SimParams, x_scaled, spec: FitSpec) -> SimParams:
