@@ -409,6 +409,94 @@ def bench_scanner(args):
     print(json.dumps(line), flush=True)
 
 
+PHANTOM_DIR = os.path.join(ROOT, "tests", "golden", "phantom")
+# the fits of record drop offsets within 1 ppm of water; the CEST pool's T2 starts at 10 ms
+PHANTOM_EXCLUDE_PPM, PHANTOM_POOL_T2 = 1.0, 0.010
+# pair checks: fitted f ratios that a shared calibration error cannot move (GAG 100 / 50 mg/mL; one creatine
+# and one carnosine solution each at two pH values)
+PHANTOM_PAIRS = (("GAG 100 / 50 mg/mL", "GAG100_pH6.31", "GAG50_pH6.20", 2.0),
+                 ("creatine 7.51 / 6.31", "creatine_pH7.51", "creatine_pH6.31", 1.0),
+                 ("carnosine 7.44 / 6.59", "carnosine_pH7.44", "carnosine_pH6.59", 1.0))
+
+
+def phantom_experiment(kind):
+    """The measured seven-tube phantom (2026-09-27, 2.8936 T) as the reference's tutorials 08 (QUESP, eight
+    B1 rungs) and 09 (QUEST, six saturation times) fit it: the six tubes with a CEST pool, each with its
+    measured water R1, B0 offset and relative B1, water R2 from the PBS tube, start magnetisation 0.557, the
+    scanner sequences converted to events (float32, reference row dropped), offsets within 1 ppm of water
+    left out of both data and model, the rungs stacked. Returns (experiment, spec, Z, noise, names, n_per_block)."""
+    from bmc_jax.fitting import Experiment, FitSpec, FreeParam
+    from bmc_jax.sequences import load_events
+    from bmc_jax.simulation.params import SimParams, initial_state
+    with np.load(os.path.join(PHANTOM_DIR, "phantom_7tubes.npz"), allow_pickle=False) as d:
+        data = {k: d[k] for k in d.files}
+    tubes = np.flatnonzero(np.isfinite(data["vial_dw"]))
+    ppm = data["ppm"]
+    keep = np.abs(ppm) >= PHANTOM_EXCLUDE_PPM
+    n = tubes.size
+    one = np.ones(n, np.float32)
+    f32 = lambda v: np.asarray(v, np.float32)
+    sp = SimParams(water_r1=f32(data["R1"][tubes]), water_r2=f32(float(data["water_r2"]) * one), water_f=f32(one),
+                   cest_k=f32(np.full((n, 1), 900.0)), cest_f=f32(np.full((n, 1), 0.008)),
+                   cest_dw=f32(data["vial_dw"][tubes][:, None]), cest_r1=f32(np.ones((n, 1))),
+                   cest_r2=f32(np.full((n, 1), 1.0 / PHANTOM_POOL_T2)), mt_k=f32(0 * one), mt_f=f32(0 * one),
+                   mt_dw=f32(0 * one), mt_r1=f32(0 * one), mt_r2=f32(one), mt_lineshape=np.zeros(n, np.int32),
+                   b0=f32(float(data["b0_T"]) * one), gamma=f32(267.5153 * one),
+                   b0_inhom=f32(data["b0_hz"][tubes] / (42.5764 * float(data["b0_T"]))),
+                   rel_b1=f32(data["rel_b1"][tubes]))
+    m_start = float(data["m_start"])
+    m0 = np.stack([initial_state(SimParams(*[np.asarray(v)[i] for v in sp]), m_start=m_start) for i in range(n)])
+    if kind == "quesp":
+        stems = [f"quesp_sim_b1_{('%g' % b).replace('.', 'p')}uT" for b in data["b1_levels_uT"]]
+        Z, noise = data["Z_quesp"], data["noise_quesp"]
+    else:
+        stems = [f"quest_sim_tsat_{('%g' % t).replace('.', 'p')}s" for t in data["tsat_s"]]
+        Z, noise = data["Z_quest"], data["noise_quest"]
+    rungs = [Experiment(sp, load_events(os.path.join(PHANTOM_DIR, f"{st}_events.npz"), keep=keep), m0, 4)
+             for st in stems]
+    spec = FitSpec([FreeParam("k", "cest_k", 0, scale=1000.0, lo=5.0, hi=2000.0, x0=300.0),
+                    FreeParam("f", "cest_f", 0, scale=0.01, lo=1e-5, hi=0.03, x0=0.008),
+                    FreeParam("r2p", "cest_r2", 0, scale=100.0, lo=10.0, hi=1000.0, x0=1.0 / PHANTOM_POOL_T2)])
+    names = [str(v) for v in data["vial_names"][tubes]]
+    return (Experiment.stack(rungs), spec, Z[tubes][:, :, keep].reshape(n, -1), noise[tubes], names,
+            int(keep.sum()))
+
+
+def phantom_pairs(f, names):
+    return [dict(pair=label, f_ratio=float(f[names.index(a)] / f[names.index(b)]), expected=want)
+            for label, a, b, want in PHANTOM_PAIRS]
+
+
+def bench_phantom(args, kind):
+    """The reference's tutorial 08 (QUESP) or 09 (QUEST) on the shipped phantom measurement: the drop-in
+    fit() with 60 LM iterations, wall time against the reference's published ~15 min (one RTX 3070)."""
+    import torch
+    from bmc_jax.fitting import fit
+    ladder, spec, Z, noise, names, npb = phantom_experiment(kind)
+    fit(ladder, spec, Z, n_iter=2, n_per_block=npb)               # warm: plan, graph, allocator
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = fit(ladder, spec, Z, n_iter=60, n_per_block=npb)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    tubes = [dict(tube=nm, k=float(res["k"][i]), f=float(res["f"][i]), t2_pool_ms=1e3 / float(res["r2p"][i]),
+                  rms_over_noise=float(res.rms[i] / noise[i]),
+                  at_bounds=[s.name for s, b in zip(spec.slots, res.at_bounds[i]) if b]) for i, nm in enumerate(names)]
+    tutorial = "08 (QUESP, 8 B1 rungs)" if kind == "quesp" else "09 (QUEST, 6 saturation times)"
+    line = {"metric": f"measured-phantom fit wall time, reference tutorial {tutorial}: 6 tubes, 3 free values, "
+                      "60 LM iterations, drop-in fit()",
+            "value": t, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * t,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": t / REF_PHANTOM_FIT_S,
+            "vs_baseline_note": "reference's published ~15 min on one RTX 3070 (examples/README.md:60-64)",
+            "dtype": "f64 simulate (f32 I/O), f64 normal equations",
+            "data": "the reference's shipped 2026-09-27 phantom scan and converted scanner sequences",
+            "config": {"workload": f"phantom {kind}", "rows": int(ladder.n_offsets),
+                       "events_per_row": int(ladder.n_events)},
+            "tubes": tubes, "pair_checks": phantom_pairs(np.asarray(res["f"]), names)}
+    print(json.dumps(line), flush=True)
+    return line
+
+
 REF_JOINT_FIT_S = 7200.0    # examples/README.md:65 of the reference: about 2 h on one RTX 3070
 
 
@@ -484,11 +572,30 @@ def bench_joint(args):
     print(json.dumps(line), flush=True)
 
 
+def host_threads():
+    """Threads for the CPU baselines: the host's cores, at most the 16 a one-GPU box grants a job."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(prm, m0, ev, nc, slots, budget_s=15.0):
     """The C oracle on this host's cores: whole spectra of the first voxels, as many as fit the budget."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_c as OC
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     ev64 = {k: np.asarray(getattr(ev, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
     O = ev64["dt"].shape[0]
     t0 = time.perf_counter()
@@ -499,9 +606,122 @@ def cpu_baseline(prm, m0, ev, nc, slots, budget_s=15.0):
     OC.simulate(prm[1:1 + n_vox].astype(np.float64), nc, ev64, m0[1:1 + n_vox].astype(np.float64), slots=slots,
                 threads=threads)
     wall = time.perf_counter() - t0
-    return dict(value=n_vox / wall, unit="Z-spectra/s (fwd+Jac)", cores=threads, kind="port",
+    return dict(value=n_vox / wall, unit="Z-spectra/s (fwd+Jac)", cores=threads, kind="port", cpu_model=cpu_model(),
                 sample=f"{n_vox} voxels x {O} offsets x {ev64['dt'].shape[1]:,} events, 5 tangents, float64 "
                        f"Pade-13 per event (no reuse), {wall:.1f} s wall on {threads} threads")
+
+
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def cpu_lm_baseline(Z, x0, n_iter=30, budget_s=15.0, rms_tol=1e-4):
+    """The reference's LM (lm.py:43-61: every voxel its own damping, all n_iter iterations, no early exit)
+    with the float64 C oracle as model and Jacobian, on this host's cores, over the first voxels of the
+    held-out ladder set: fitted voxels per second (rms at most rms_tol), extrapolated from the sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_c as OC
+    from bmc_jax import learned
+    from bmc_jax.simulation.params import initial_state, pack
+    threads = host_threads()
+    spec = learned.fit_spec()
+    base = learned.base_params(1, dtype=np.float64)
+    rec = pack(base, 1, dtype=np.float64)[0]
+    m0 = np.asarray(initial_state(base), np.float64).reshape(-1)
+    slots = [_slot(f, i) for f, i in (("cest_k", 0), ("cest_f", 0), ("water_r1", None), ("water_r2", None),
+                                      ("b0_inhom", None))]
+    ev = learned.ladder_events()
+    ev64 = {k: np.asarray(getattr(ev, k)) for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    sc = spec.scales
+    lo, hi = spec.lo_phys / sc, spec.hi_phys / sc
+
+    def model(x, want_j=True):
+        prm = np.repeat(rec[None], x.shape[0], axis=0)
+        prm[:, slots] = x * sc
+        m, dm = OC.simulate(prm, 1, ev64, m0, slots=slots if want_j else (), threads=threads)
+        return m[:, :, 4], (dm[:, :, :, 4] * sc if want_j else None)
+
+    t0 = time.perf_counter()
+    model(x0[:1] / sc)
+    one = time.perf_counter() - t0
+    V = int(min(len(Z), max(8, budget_s / max(2 * n_iter * one, 1e-9))))
+    z = np.asarray(Z[:V], np.float64)
+    x = np.asarray(x0[:V], np.float64) / sc
+    lam = np.full(V, 1e-2)
+    eye = np.eye(len(slots))
+    t0 = time.perf_counter()
+    for _ in range(n_iter):
+        zm, J = model(x)
+        r = zm - z
+        c = np.einsum("vo,vo->v", r, r)
+        H = np.einsum("vop,voq->vpq", J, J)
+        g = np.einsum("vop,vo->vp", J, r)
+        A = H + lam[:, None, None] * (H * eye + 1e-9 * eye)
+        xt = np.clip(x + np.linalg.solve(A, -g[..., None])[..., 0], lo, hi)
+        zt, _ = model(xt, want_j=False)
+        ct = np.einsum("vo,vo->v", zt - z, zt - z)
+        ok = ct < c
+        x = np.where(ok[:, None], xt, x)
+        lam = np.where(ok, lam * 0.5, lam * 3.0)
+    wall = time.perf_counter() - t0
+    rms = np.sqrt(np.minimum(ct, c) / z.shape[1])
+    fitted = int(np.sum(rms <= rms_tol))
+    return dict(value=fitted / wall, unit="fitted voxels/s (plain-x0 LM, all 30 iterations)", cores=threads,
+                kind="port", cpu_model=cpu_model(),
+                sample=f"{V} held-out voxels x 244 CW rows, 5 free values, 30 iterations of the reference LM with "
+                       f"the float64 C oracle (Pade-13, forward-mode tangents) as model, {wall:.1f} s on {threads} "
+                       f"threads, {fitted} of {V} at rms <= {rms_tol:g}")
+
+
+def _slot(field, index):
+    from bmc_jax import _native
+    return _native.slot_index(field, index, 1)
+
+
+def nn_record(args, log=None):
+    """The learned-proposal comparison (NN-started LM against plain-x0 LM, both early-exit and all 30
+    iterations) on the 262,144-voxel held-out ladder set, with the MLP's own roofline and a CPU baseline."""
+    import torch
+    from bmc_jax import _native, learned
+    r = learned.benchmark(n_train=args.nn_train, n_test=args.nn_test, epochs=args.nn_epochs, log=log)
+    prop = r.pop("proposal")
+    nf = r["noise_free"]
+    # the MLP's GEMMs at their roofline: inference over the held-out spectra, timed on the stream
+    Zt = r.pop("Z_noise_free")
+    stream = torch.cuda.current_stream()
+    prop.mlp.forward(Zt)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    prop.mlp.forward(Zt)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_inf = 1e-3 * e0.elapsed_time(e1)
+    dims = learned.DIMS
+    mac = sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
+    inf_tflops = 2.0 * mac * Zt.shape[0] / t_inf / 1e12
+    train_flops = 3 * 2.0 * mac * (args.nn_train // 4096) * 4096 * args.nn_epochs
+    out = {
+        "metric": "fitted voxels/s (NN-initialised LM vs plain LM, rms <= 1e-4, noise-free held-out ladders)",
+        "value": nf["nn"]["fitted_per_s"], "unit": "fitted voxels/s",
+        "speedup_vs_plain_all_iterations": nf["speedup_vs_fixed"], "speedup_vs_plain_early_exit": nf["speedup"],
+        "nn_lm": nf["nn"], "plain_lm_all_iterations": nf["plain_fixed"], "plain_lm_early_exit": nf["plain"],
+        "noisy_sigma_1e-3_rms_le_1.15_sigma": r["noisy"],
+        "config": {"workload": "learned LM proposal: 2-pool CW QUESP ladder (0.5, 1, 2, 4 uT, 2 s), 61 offsets, "
+                               "MLP 244-512-512-256-5 GELU, input standardised", "n_train": args.nn_train,
+                   "epochs": args.nn_epochs, "n_test": args.nn_test},
+        "train_seconds": r["train_seconds"], "final_train_loss": r["final_train_loss"],
+        "roofline": {"bound": "mfma", "kernel": "gemm_nt_kernel (bf16 MFMA 16x16x32, fp32 accumulate)",
+                     "achieved": inf_tflops, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": inf_tflops / PEAK_BF16_TFLOPS,
+                     "measured_on": f"MLP inference over {Zt.shape[0]} spectra (batches of 4096), "
+                                    f"{1e3 * t_inf:.2f} ms including the input standardisation tiles",
+                     "training_tflops": train_flops / max(r["train_seconds"], 1e-9) / 1e12},
+    }
+    if not args.no_cpu:
+        rng = np.random.default_rng(55)
+        x_true, _ = learned.draw_params(64, rng)
+        out["cpu_baseline"] = cpu_lm_baseline(learned.simulate_ladder(x_true),
+                                              np.broadcast_to(learned.fit_spec().x0_phys, (64, 5)))
+    return out
 
 
 def main():
@@ -517,11 +737,13 @@ def main():
     ap.add_argument("--image-voxels", type=int, default=1024)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer (the reference point for scaling HBM counters)")
-    ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven", "scanner", "joint"), default="pulsed",
+    ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven", "scanner", "joint", "tutorial08",
+                                           "tutorial09"), default="pulsed",
                     help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
     ap.add_argument("--nn-train", type=int, default=2_000_000)
     ap.add_argument("--nn-test", type=int, default=262_144)
     ap.add_argument("--nn-epochs", type=int, default=30)
+    ap.add_argument("--no-nn", action="store_true", help="leave the learned-proposal sub-record out of the line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (voxels are independent: "
@@ -549,6 +771,10 @@ def main():
         _native.torch_cuda()
         bench_scanner(args)
         return
+    if args.workload in ("tutorial08", "tutorial09"):
+        _native.torch_cuda()
+        bench_phantom(args, "quesp" if args.workload == "tutorial08" else "quest")
+        return
     if args.workload == "joint":
         _native.torch_cuda()
         bench_joint(args)
@@ -558,21 +784,13 @@ def main():
         bench_design(args)
         return
     if args.workload == "nn":
-        from bmc_jax import learned
         _native.torch_cuda()
-        r = learned.benchmark(n_train=args.nn_train, n_test=args.nn_test, epochs=args.nn_epochs,
-                              log=lambda m: print(m, file=sys.stderr, flush=True))
-        nf, nz = r["noise_free"], r["noisy"]
-        out = {"metric": "fitted voxels/s (NN-initialised LM with plain fallback, rms <= 1e-4, noise-free)",
-               "value": nf["nn"]["fitted_per_s"], "unit": "fitted voxels/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-               "ms_per_step": 1e3 * nf["nn"]["seconds"], "higher_is_better": True, "scaling": "weak",
-               "vs_baseline": None, "dtype": "f64 simulate / bf16-fp32 MLP",
-               "data": "synthetic (simulator-generated, seeded)",
-               "config": {"workload": "learned LM proposal: 2-pool CW QUESP ladder (0.5, 1, 2, 4 uT, 2 s), 61 offsets, "
-                                      "MLP 244-512-512-256-5 GELU", "n_train": args.nn_train, "n_test": args.nn_test},
-               "plain_lm_early_exit": nf["plain"], "plain_lm_all_iterations": nf["plain_fixed"], "nn_lm": nf["nn"],
-               "speedup_vs_plain_early_exit": nf["speedup"], "speedup_vs_plain_all_iterations": nf["speedup_vs_fixed"],
-               "noisy_sigma_1e-3_rms_le_1.15_sigma": nz, "train_seconds": r["train_seconds"]}
+        rec = nn_record(args, log=lambda m: print(m, file=sys.stderr, flush=True))
+        out = {"metric": rec.pop("metric"), "value": rec.pop("value"), "unit": rec.pop("unit"), "n_gpus": 1,
+               "steps": 1, "warmup": 0, "ms_per_step": 1e3 * rec["nn_lm"]["seconds"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64 simulate / bf16-fp32 MLP",
+               "data": "synthetic (simulator-generated, seeded)"}
+        out.update(rec)
         print(json.dumps(out), flush=True)
         return
 
@@ -647,6 +865,10 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prm, m0, ev, nc, slots)
+    if not args.no_nn:
+        del m, J
+        torch.cuda.empty_cache()
+        out["nn"] = nn_record(args, log=lambda msg: print(msg, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 

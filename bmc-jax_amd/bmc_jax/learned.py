@@ -150,6 +150,27 @@ class MLP:
         _native.check(_native.lib().bmc_mlp_set_params(self._h, w.ctypes.data_as(ctypes.c_void_p)),
                       "bmc_mlp_set_params")
 
+    def fit_input_norm(self, x):
+        """Standardise the inputs by x's column means and standard deviations (x: device float32 [n, dims[0]])."""
+        x = x.contiguous()
+        _native.check(_native.lib().bmc_mlp_fit_input_norm(self._h, ctypes.c_void_p(x.data_ptr()), int(x.shape[0]),
+                                                           _native.stream_ptr()), "bmc_mlp_fit_input_norm")
+
+    def input_norm(self):
+        """(shift, scale) [dims[0]] float32, NumPy."""
+        sh, sc = np.zeros(self.dims[0], np.float32), np.zeros(self.dims[0], np.float32)
+        _native.check(_native.lib().bmc_mlp_get_input_norm(self._h, sh.ctypes.data_as(ctypes.c_void_p),
+                                                           sc.ctypes.data_as(ctypes.c_void_p)), "bmc_mlp_get_input_norm")
+        return sh, sc
+
+    def set_input_norm(self, shift, scale):
+        if shift is None:
+            _native.check(_native.lib().bmc_mlp_set_input_norm(self._h, None, None), "bmc_mlp_set_input_norm")
+            return
+        sh, sc = np.ascontiguousarray(shift, np.float32), np.ascontiguousarray(scale, np.float32)
+        _native.check(_native.lib().bmc_mlp_set_input_norm(self._h, sh.ctypes.data_as(ctypes.c_void_p),
+                                                           sc.ctypes.data_as(ctypes.c_void_p)), "bmc_mlp_set_input_norm")
+
     def forward(self, x):
         """x: device float32 tensor [n, dims[0]] -> device tensor [n, dims[-1]] (chunked by max_batch)."""
         torch = _native.torch_cuda()
@@ -223,8 +244,12 @@ def _bind(L):
     L.bmc_mlp_forward.argtypes = [vp, vp, ctypes.c_int, vp, vp]
     L.bmc_mlp_gradients.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_float), vp]
     L.bmc_mlp_train_step.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_float, vp, vp]
+    L.bmc_mlp_fit_input_norm.argtypes = [vp, vp, ctypes.c_int64, vp]
+    L.bmc_mlp_set_input_norm.argtypes = [vp, vp, vp]
+    L.bmc_mlp_get_input_norm.argtypes = [vp, vp, vp]
     for n in ("bmc_mlp_create", "bmc_mlp_destroy", "bmc_mlp_set_params", "bmc_mlp_get_params", "bmc_mlp_forward",
-              "bmc_mlp_gradients", "bmc_mlp_train_step"):
+              "bmc_mlp_gradients", "bmc_mlp_train_step", "bmc_mlp_fit_input_norm", "bmc_mlp_set_input_norm",
+              "bmc_mlp_get_input_norm"):
         getattr(L, n).restype = ctypes.c_int
     _BOUND = True
 
@@ -249,6 +274,7 @@ def train_proposal(n_train=2_000_000, epochs=30, batch=4096, lr=1e-3, noise=1e-3
     Z = Z + noise * torch.randn(Z.shape, generator=gen, device="cuda", dtype=torch.float32)
     T = torch.as_tensor(u.astype(np.float32), device="cuda")
     mlp = MLP(max_batch=batch, seed=seed)
+    mlp.fit_input_norm(Z)
     hist = mlp.train(Z, T, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
     torch.cuda.synchronize()
     return Proposal(mlp, hist, time.perf_counter() - t0)
@@ -288,10 +314,11 @@ class DeviceLadder:
     def fit(self, x0_phys, idx=None, n_iter=30, rms_tol=1e-4, early_exit=True, with_iters=False):
         """LM from x0_phys [m, 5] (device) for the voxels idx (all if None). Returns (x_phys, rms) on the device
         (and the iterations each voxel ran with with_iters). With early_exit a voxel stops once its rms is at
-        most rms_tol, also before its first step when its start already is (bmc_lm_fit's start check)."""
+        most rms_tol, also before its first step when its start already is (checked on the first Jacobian
+        sweep's own state, so no separate sweep is spent on the check)."""
         torch = _native.torch_cuda()
         sel = (lambda t: t) if idx is None else (lambda t: t[idx].contiguous())
-        flags = (_native.LM_EARLY_EXIT | _native.LM_START_CHECK if early_exit else 0) | _native.LM_GRAPH
+        flags = (_native.LM_EARLY_EXIT if early_exit else 0) | _native.LM_GRAPH
         x, ctraj, _, nd = _native.lm_fit(self.plan, sel(self.prm), 1, _native.BMC_F32, sel(self.m0), self.slots,
                                          self.spec.scales, self.lo, self.hi, 4, sel(self.Z),
                                          (x0_phys.to(torch.float64) / self.scales_t), n_iter, flags=flags,
@@ -346,6 +373,8 @@ def benchmark(n_train=2_000_000, n_test=262_144, epochs=30, n_iter=30, rms_tol=1
     for name, Zs, tol in (("noise_free", Z, rms_tol), ("noisy", Zn, 1.15 * noise)):
         out[name] = _compare_starts(prop, Zs, xt, n_iter, tol)
         out[name]["rms_tol"] = tol
+    out["proposal"] = prop
+    out["Z_noise_free"] = Z
     return out
 
 

@@ -488,6 +488,24 @@ int fill_sim_args(const bmc_plan* plan, int V, int nc, int dtype, const void* pa
 #endif
 int tangents_per_pass(int P) { return P <= 0 ? 0 : (P < BMC_TANGENTS_PER_PASS ? P : BMC_TANGENTS_PER_PASS); }
 
+int stream_after_null(hipStream_t* out) {
+  *out = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t ready = nullptr;
+  BMC_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipError_t e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(ready, nullptr);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, ready, 0);
+  if (ready) (void)hipEventDestroy(ready);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(s);
+    set_error(std::string("ordering a stream after the null stream: ") + hipGetErrorString(e));
+    return BMC_EHIP;
+  }
+  *out = s;
+  return BMC_OK;
+}
+
 }  // namespace bmc
 
 extern "C" int bmc_tangents_per_pass(int n_tangents) { return tangents_per_pass(n_tangents); }

@@ -119,6 +119,12 @@ bool use_cwj(const Plan& p, const SimArgs& a, bool forward_mode);
 int launch_cwj(int nc, const SimArgs& a, const int32_t* row_live, hipStream_t stream);
 // Tangents carried per simulate pass for P tangents (register budget of the kernel).
 int tangents_per_pass(int P);
+// A stream of the library's own for work a caller issued on the null stream but that has to run on a
+// capturable stream (a graph-captured LM iteration): non-blocking, and made to wait for everything
+// already queued on the null stream, so that the caller's earlier writes to its buffers (and any
+// earlier use of memory its allocator hands out again) complete first. The caller synchronises it
+// before returning.
+int stream_after_null(hipStream_t* out);
 
 }  // namespace bmc
 

@@ -52,7 +52,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 struct NeArgs {
-  int V, O, P, dtype, has_free;
+  int V, O, P, dtype, has_free, early;
+  double rms_tol;
   const double* z;
   const double* J;
   const void* zmeas;
@@ -81,6 +82,16 @@ __global__ void __launch_bounds__(64) normal_eq_kernel(NeArgs a) {
     c = fma(r, r, c);
   }
   c = wave_sum(c);
+  if (a.early && sqrt(c / (double)O) <= a.rms_tol) {
+    // with early exit, a problem already within the tolerance at the point its Jacobian was just taken
+    // at (its start, in the first iteration) takes no step and stops
+    if (lane == 0) {
+      a.s.cost[v] = c;
+      a.s.active[v] = 0;
+      a.s.jac_live[v] = 0;
+    }
+    return;
+  }
   for (int p = 0; p < P; p++) {
     double acc = 0.0;
     for (int o = lane; o < O; o += 64) {
@@ -346,6 +357,8 @@ int lm_run(const bmc_plan* plan, int V, int n_cest, int dtype, const void* param
   ne.P = P;
   ne.dtype = dtype;
   ne.has_free = free_dev != nullptr;
+  ne.early = (flags & BMC_LM_EARLY_EXIT) ? 1 : 0;
+  ne.rms_tol = early_rms;
   ne.z = w.z;
   ne.J = w.J;
   ne.zmeas = z_meas_dev;
@@ -471,8 +484,8 @@ extern "C" int bmc_lm_fit(const bmc_plan* plan, int n_voxels, int n_cest, int dt
   if (V == 0 || n_iter == 0) return BMC_OK;
   hipStream_t stream = (hipStream_t)stream_;
   hipStream_t own = nullptr;
-  if ((flags & BMC_LM_GRAPH) && stream == nullptr) {  // a graph is captured on a stream of its own
-    BMC_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+  if ((flags & BMC_LM_GRAPH) && stream == nullptr) {  // the null stream cannot be captured
+    if ((rc = stream_after_null(&own)) != BMC_OK) return rc;
     stream = own;
   }
   struct Guard {

@@ -1,16 +1,23 @@
 // The learned Levenberg-Marquardt proposal: a small MLP trained and run with hand-written BF16 MFMA
 // GEMMs (FP32 accumulate) on CDNA4. No counterpart in the reference; it supplies the LM start x0.
 //
-// Layout: activations row-major [batch, features]; weights W_l [out, in] (FP32 master copy plus a BF16
-// copy for the matrix cores), biases [out]. Hidden layers use GELU (0.5 x (1 + erf(x / sqrt 2))), the
-// last layer is linear; the loss is the mean over batch and outputs of the squared error.
+// Layout: activations row-major [batch, features]; weights W_l [out, in] (FP32 master copy), biases [out].
+// Hidden layers use GELU (0.5 x (1 + erf(x / sqrt 2))), the last layer is linear; the loss is the mean
+// over batch and outputs of the squared error. The input is standardised per feature before the first
+// GEMM, (x - shift) * scale, with shift / scale fitted to the training set (bmc_mlp_fit_input_norm).
 //
-// GEMM: one kernel for C[M, N] = A[M, K] B[K, N] with arbitrary strides on A and B (so the forward
-// X W^T, the input gradient dY W and the weight gradient dY^T X are the same kernel). A 256-thread block
-// computes a 64 x 64 tile; each wave a 32 x 32 quarter as 2 x 2 tiles of v_mfma_f32_16x16x16_bf16. K is
-// walked 32 at a time through LDS. Lane layout of the 16x16x16 BF16 MFMA (64 lanes, l = lane):
-//   A fragment: row l % 16, k = 4 (l / 16) .. +3       B fragment: column l % 16, k = 4 (l / 16) .. +3
-//   D fragment: column l % 16, rows 4 (l / 16) .. +3
+// GEMM: every product is C[M, N] = A[M, K] B^T with both A [M, K] and B [N, K] K-contiguous BF16, so that
+// operands are staged with 16-byte loads along K. Each BF16 activation and weight is therefore kept in
+// the two orientations the products need: activations [batch, features] (forward, input gradient) and
+// [features, batch] (weight gradient), weights [out, in] (forward) and [in, out] (input gradient); the
+// transposed copies are written by LDS-tiled epilogue kernels. Leading dimensions are padded to 8
+// elements so every 16-byte load is aligned; reads past the logical K are masked to zero.
+//
+// The kernel: 256 threads (2 x 2 waves) per BM x BN tile (128 x 128 or 64 x 64), K in slabs of 64
+// staged through double-buffered LDS, v_mfma_f32_16x16x32_bf16 (lane l: A row l % 16 with k = 8 (l / 16)
+// .. + 7, B column l % 16 with the same k, D column l % 16 with rows 4 (l / 16) .. + 3). Products with too
+// few tiles to fill 256 CUs (the weight gradients, K = batch) split K over the grid's z axis into FP32
+// partials that a second kernel adds in a fixed order (training stays bitwise reproducible).
 #include <math.h>
 
 #include <algorithm>
@@ -24,7 +31,7 @@
 namespace bmc {
 namespace mlp {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -34,24 +41,37 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
 }
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
-constexpr int BM = 64, BN = 64, BK = 32;
+inline long pad8(long n) { return (n + 7) & ~7L; }
+
+constexpr int BK = 64;         // K per LDS slab
+constexpr int LDS_ROW = BK + 8;  // bf16 per LDS row: 144 bytes, so the 16 rows a fragment read touches
+                                 // fall on distinct banks
+
+// Epilogue of one output element (applied by the GEMM when K is not split, else by the reduction).
+struct Epi {
+  float* C;          // [M, ldc] fp32 or null
+  long ldc;
+  float alpha;
+  int accumulate;    // C += result
+  const float* bias; // [N] or null
+  int act;           // 0 none, 1 GELU
+  float* Cpre;       // pre-activation [M, ldc] or null
+  uint16_t* Cbf;     // bf16 [M, ldcb] or null
+  long ldcb;
+  uint16_t* CbfT;    // bf16 transposed [N, ldct] or null
+  long ldct;
+};
 
 struct GemmArgs {
   int M, N, K;
-  const uint16_t* A;  // bf16
-  long sam, sak;      // A[m, k] = A[m * sam + k * sak]
-  const uint16_t* B;  // bf16
-  long sbk, sbn;      // B[k, n] = B[k * sbk + n * sbn]
-  float* C;           // [M, ldc]
-  long ldc;
-  const float* bias;  // [N] or null
-  int act;            // 0 none, 1 GELU
-  float* Cpre;        // pre-activation copy [M, ldc] or null
-  uint16_t* Cbf;      // bf16 copy of C [M, ldc] or null
-  float alpha;
-  int accumulate;     // C += result
+  const uint16_t* A;  // [M, lda] bf16, K-contiguous
+  long lda;
+  const uint16_t* B;  // [N, ldb] bf16, K-contiguous
+  long ldb;
+  int splits, kchunk; // K split over gridDim.z in chunks of kchunk (a multiple of BK)
+  float* part;        // [splits, M, N] fp32 partials when splits > 1
+  Epi e;
 };
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
@@ -59,124 +79,264 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
 
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-  // LDS tiles, padded by 4 bf16 per row against bank conflicts
-  __shared__ uint16_t sA[BM][BK + 4];
-  __shared__ uint16_t sB[BN][BK + 4];  // stored transposed: sB[n][k]
-  const int tid = threadIdx.x;
-  const int wave = tid / 64, lane = tid % 64;
+// v (the accumulated product) -> every output the epilogue asks for, except the transposed bf16 copy.
+__device__ __forceinline__ float finish(const Epi& e, int m, int n, float v) {
+  v *= e.alpha;
+  const long o = (long)m * e.ldc + n;
+  if (e.accumulate) v += e.C[o];
+  if (e.bias) v += e.bias[n];
+  if (e.Cpre) e.Cpre[o] = v;
+  if (e.act == 1) v = gelu(v);
+  if (e.C) e.C[o] = v;
+  if (e.Cbf) e.Cbf[(long)m * e.ldcb + n] = f2bf(v);
+  return v;
+}
+
+// 8 bf16 of row `row` from k, zero past K (and past the last row).
+__device__ __forceinline__ uint4 load8(const uint16_t* base, long ld, int row, int rows, int k, int K) {
+  if (row >= rows || k >= K) return make_uint4(0u, 0u, 0u, 0u);
+  const uint16_t* p = base + (long)row * ld + k;
+  if (k + 8 <= K) return *reinterpret_cast<const uint4*>(p);
+  uint16_t t[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) t[i] = (k + i < K) ? p[i] : (uint16_t)0;
+  return make_uint4(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16), t[4] | ((uint32_t)t[5] << 16),
+                    t[6] | ((uint32_t)t[7] << 16));
+}
+
+template <int BM, int BN>
+__global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
+  constexpr int FM = BM / 32, FN = BN / 32;        // 16 x 16 fragments per wave (wave tile BM/2 x BN/2)
+  constexpr int CA = BM * BK / 8 / 256, CB = BN * BK / 8 / 256;  // 16-byte chunks per thread per slab
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * LDS_ROW];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * LDS_ROW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
-  floatx4 acc[2][2];
+  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+  const int kbeg = blockIdx.z * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  floatx4 acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < FM; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    // stage A[m0:m0+64, k0:k0+32] and B[k0:k0+32, n0:n0+64]: 2048 elements each, 8 per thread
+    for (int j = 0; j < FN; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  uint4 ra[CA], rb[CB];
+  auto fetch = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-      const int e = tid + r * 256;
-      const int mi = e / BK, ki = e % BK;
-      const int gm = m0 + mi, gk = k0 + ki;
-      sA[mi][ki] = (gm < g.M && gk < g.K) ? g.A[gm * g.sam + gk * g.sak] : (uint16_t)0;
-      const int ni = e / BK, kj = e % BK;
-      const int gn = n0 + ni, gk2 = k0 + kj;
-      sB[ni][kj] = (gn < g.N && gk2 < g.K) ? g.B[gk2 * g.sbk + gn * g.sbn] : (uint16_t)0;
+    for (int c = 0; c < CA; c++) {
+      const int e = tid + c * 256, row = e >> 3, kc = (e & 7) * 8;
+      ra[c] = load8(g.A, g.lda, m0 + row, g.M, k0 + kc, kend);
     }
-    __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 16) {
-      const int kb = kk + 4 * (lane / 16);
-      bf16x4 a[2], b[2];
-#pragma unroll
-      for (int i = 0; i < 2; i++) {
-        const int row = wm + 16 * i + (lane % 16);
-        a[i] = bf16x4{(short)sA[row][kb], (short)sA[row][kb + 1], (short)sA[row][kb + 2], (short)sA[row][kb + 3]};
-      }
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int col = wn + 16 * j + (lane % 16);
-        b[j] = bf16x4{(short)sB[col][kb], (short)sB[col][kb + 1], (short)sB[col][kb + 2], (short)sB[col][kb + 3]};
-      }
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    for (int c = 0; c < CB; c++) {
+      const int e = tid + c * 256, row = e >> 3, kc = (e & 7) * 8;
+      rb[c] = load8(g.B, g.ldb, n0 + row, g.N, k0 + kc, kend);
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < CA; c++) {
+      const int e = tid + c * 256, row = e >> 3, kc = (e & 7) * 8;
+      *reinterpret_cast<uint4*>(&sA[buf][row * LDS_ROW + kc]) = ra[c];
+    }
+#pragma unroll
+    for (int c = 0; c < CB; c++) {
+      const int e = tid + c * 256, row = e >> 3, kc = (e & 7) * 8;
+      *reinterpret_cast<uint4*>(&sB[buf][row * LDS_ROW + kc]) = rb[c];
+    }
+  };
+
+  int buf = 0;
+  if (kbeg < kend) {
+    fetch(kbeg);
+    stash(0);
   }
-  // epilogue
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
+    if (more) fetch(k0 + BK);  // the next slab's global loads overlap this slab's MFMAs
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+    for (int ks = 0; ks < BK; ks += 32) {
+      const int kk = ks + 8 * (lane >> 4);
+      bf16x8 a[FM], b[FN];
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+      for (int i = 0; i < FM; i++)
+        a[i] = *reinterpret_cast<const bf16x8*>(&sA[buf][(wm + 16 * i + (lane & 15)) * LDS_ROW + kk]);
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int gm = m0 + wm + 16 * i + 4 * (lane / 16) + r;
-        const int gn = n0 + wn + 16 * j + (lane % 16);
-        if (gm < g.M && gn < g.N) {
-          float v = g.alpha * acc[i][j][r];
-          const long o = gm * g.ldc + gn;
-          if (g.accumulate) v += g.C[o];
-          if (g.bias) v += g.bias[gn];
-          if (g.Cpre) g.Cpre[o] = v;
-          if (g.act == 1) v = gelu(v);
-          g.C[o] = v;
-          if (g.Cbf) g.Cbf[o] = f2bf(v);
+      for (int j = 0; j < FN; j++)
+        b[j] = *reinterpret_cast<const bf16x8*>(&sB[buf][(wn + 16 * j + (lane & 15)) * LDS_ROW + kk]);
+#pragma unroll
+      for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) stash(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+  if (g.splits > 1) {
+    float* P = g.part + (long)blockIdx.z * g.M * g.N;
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+      for (int j = 0; j < FN; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r, n = n0 + wn + 16 * j + (lane & 15);
+          if (m < g.M && n < g.N) P[(long)m * g.N + n] = acc[i][j][r];
+        }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++) {
+      const int mb = m0 + wm + 16 * i + 4 * (lane >> 4), n = n0 + wn + 16 * j + (lane & 15);
+      if (n >= g.N) continue;
+      uint16_t t[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) t[r] = (mb + r < g.M) ? f2bf(finish(g.e, mb + r, n, acc[i][j][r])) : (uint16_t)0;
+      if (g.e.CbfT) {
+        uint16_t* q = g.e.CbfT + (long)n * g.e.ldct + mb;
+        if (mb + 4 <= g.M) {  // four consecutive rows of column n: one 8-byte store
+          *reinterpret_cast<uint2*>(q) = make_uint2(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16));
+        } else {
+          for (int r = 0; r < 4 && mb + r < g.M; r++) q[r] = t[r];
         }
       }
+    }
 }
 
-__global__ void f32_to_bf16_kernel(const float* x, uint16_t* y, long n) {
+// Sum of the split-K partials in a fixed order, then the epilogue.
+__global__ void __launch_bounds__(256) gemm_reduce_kernel(GemmArgs g) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = f2bf(x[i]);
+  if (i >= (long)g.M * g.N) return;
+  const int m = (int)(i / g.N), n = (int)(i % g.N);
+  float v = 0.f;
+  for (int s = 0; s < g.splits; s++) v += g.part[(long)s * g.M * g.N + i];
+  v = finish(g.e, m, n, v);
+  if (g.e.CbfT) g.e.CbfT[(long)n * g.e.ldct + m] = f2bf(v);
 }
 
-// dL/dY of the mean squared error, L = mean((Y - T)^2) over batch x outputs; also accumulates L.
-__global__ void mse_grad_kernel(const float* Y, const float* T, int B, int D, float* dY, uint16_t* dYbf,
-                                float* loss) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  float l = 0.f;
-  if (i < (long)B * D) {
-    const float d = Y[i] - T[i];
-    const float gr = 2.0f * d / (float)((long)B * D);
-    dY[i] = gr;
-    dYbf[i] = f2bf(gr);
-    l = d * d / (float)((long)B * D);
+// Elementwise producers of a GEMM operand, over 64 x 64 tiles of an fp32 [rows, cols] matrix, writing
+// the bf16 copy [rows, ldo] and its transpose [cols, ldt] (both through LDS, so both stores coalesce):
+//   TILE_INPUT  v = (x - shift[c]) * scale[c]    (shift / scale null: v = x; also packs the weights)
+//   TILE_GELU   v = x * gelu'(pre), v also written back over x in fp32 (dZ of a hidden layer)
+//   TILE_MSE    v = 2 (x - t) / (rows cols), v written to out32; the loss mean((x - t)^2) into *loss
+// With colsum set (TILE_GELU, TILE_MSE: v is a layer's dZ), each tile also writes the column sums of its
+// 64 rows of v, summed in a fixed order, to colsum[tile row][cols]: the bias gradient's partial sums.
+enum { TILE_INPUT = 0, TILE_GELU = 1, TILE_MSE = 2 };
+
+struct TileArgs {
+  int mode, rows, cols;
+  const float* x;     // [rows, ldx]
+  long ldx;
+  const float* aux;   // shift [cols] (input), pre [rows, ldx] (gelu), t [rows, ldx] (mse)
+  const float* scale; // [cols] (input) or null
+  float* out32;       // fp32 result [rows, ldx] or null
+  uint16_t* bf;       // [rows, ldo] or null
+  long ldo;
+  uint16_t* bfT;      // [cols, ldt] or null
+  long ldt;
+  float* loss;
+  float* colsum;      // [ceil(rows / 64), cols] or null
+};
+
+__global__ void __launch_bounds__(256) tile_kernel(TileArgs a) {
+  __shared__ uint16_t t[64][66];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  float lsum = 0.f, csum = 0.f;
+  const float inv = 1.0f / ((float)a.rows * (float)a.cols);
+#pragma unroll 4
+  for (int k = 0; k < 16; k++) {
+    const int r = r0 + 4 * k + ty, c = c0 + tx;
+    uint16_t h = 0;
+    if (r < a.rows && c < a.cols) {
+      const long o = (long)r * a.ldx + c;
+      const float x = a.x[o];
+      float v;
+      if (a.mode == TILE_INPUT) {
+        v = a.aux ? (x - a.aux[c]) * a.scale[c] : x;
+      } else if (a.mode == TILE_GELU) {
+        v = x * gelu_grad(a.aux[o]);
+        a.out32[o] = v;
+      } else {
+        const float d = x - a.aux[o];
+        v = 2.0f * d * inv;
+        a.out32[o] = v;
+        lsum += d * d * inv;
+      }
+      if (a.mode != TILE_INPUT) csum += v;
+      h = f2bf(v);
+      if (a.bf) a.bf[(long)r * a.ldo + c] = h;
+    }
+    t[4 * k + ty][tx] = h;
   }
-  for (int m = 32; m > 0; m >>= 1) l += __shfl_xor(l, m, 64);
-  if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, l);
-}
-
-// dZ = dA * gelu'(pre) (elementwise), with a bf16 copy for the next GEMM.
-__global__ void gelu_back_kernel(const float* dA, const float* pre, long n, float* dZ, uint16_t* dZbf) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const float v = dA[i] * gelu_grad(pre[i]);
-    dZ[i] = v;
-    dZbf[i] = f2bf(v);
+  if (a.mode == TILE_MSE && a.loss) {
+    for (int m = 32; m > 0; m >>= 1) lsum += __shfl_xor(lsum, m, 64);
+    if (tx == 0) atomicAdd(a.loss, lsum);
   }
-}
-
-// db[n] = sum_b dZ[b, n] in two passes with a fixed summation order (bitwise reproducible training):
-// partial sums over chunks of kBiasRows rows, 64 columns x 4 row lanes per block so that every row is
-// read in 256-byte runs, then one thread per column adds the chunks.
-constexpr int kBiasRows = 256;
-
-__global__ void __launch_bounds__(256) bias_grad_partial_kernel(const float* dZ, int B, int N, float* part) {
-  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int r1 = min(B, (int)(blockIdx.y + 1) * kBiasRows);
-  float s = 0.f;
-  if (c < N)
-    for (int r = blockIdx.y * kBiasRows + (threadIdx.x >> 6); r < r1; r += 4) s += dZ[(long)r * N + c];
-  __shared__ float sh[4][64];
-  sh[threadIdx.x >> 6][threadIdx.x & 63] = s;
+  if (a.colsum) {
+    __shared__ float cs[4][64];
+    cs[ty][tx] = csum;
+    __syncthreads();
+    if (ty == 0 && c0 + tx < a.cols)
+      a.colsum[(long)blockIdx.y * a.cols + c0 + tx] = (cs[0][tx] + cs[1][tx]) + (cs[2][tx] + cs[3][tx]);
+  }
+  if (!a.bfT) return;
   __syncthreads();
-  if (threadIdx.x < 64 && c < N)
-    part[(long)blockIdx.y * N + c] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+#pragma unroll 4
+  for (int k = 0; k < 16; k++) {
+    const int c = c0 + 4 * k + ty, r = r0 + tx;
+    if (c < a.cols && r < a.rows) a.bfT[(long)c * a.ldt + r] = t[tx][4 * k + ty];
+  }
 }
 
+// Column mean and standard deviation of x [n, d] in float64 (the input standardisation), in two passes
+// with a fixed summation order: partial sums over chunks of rows (64 columns x 4 row lanes per block, every
+// row read in 256-byte runs), then one thread per column adds the chunks.
+constexpr int kStatRows = 2048;
+
+__global__ void __launch_bounds__(256) column_stats_partial_kernel(const float* x, long n, int d, double* part) {
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
+  const long r0 = (long)blockIdx.y * kStatRows, r1 = min(n, r0 + kStatRows);
+  double s = 0.0, s2 = 0.0;
+  if (c < d)
+    for (long r = r0 + q; r < r1; r += 4) {
+      const double v = x[r * d + c];
+      s += v;
+      s2 += v * v;
+    }
+  __shared__ double sh[2][4][64];
+  sh[0][q][threadIdx.x & 63] = s;
+  sh[1][q][threadIdx.x & 63] = s2;
+  __syncthreads();
+  if (q == 0 && c < d) {
+    const int j = threadIdx.x & 63;
+    part[((long)blockIdx.y * 2 + 0) * d + c] = (sh[0][0][j] + sh[0][1][j]) + (sh[0][2][j] + sh[0][3][j]);
+    part[((long)blockIdx.y * 2 + 1) * d + c] = (sh[1][0][j] + sh[1][1][j]) + (sh[1][2][j] + sh[1][3][j]);
+  }
+}
+
+__global__ void column_stats_final_kernel(const double* part, int chunks, long n, int d, float* shift, float* scale) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d) return;
+  double S = 0.0, S2 = 0.0;
+  for (int k = 0; k < chunks; k++) {
+    S += part[((long)k * 2 + 0) * d + c];
+    S2 += part[((long)k * 2 + 1) * d + c];
+  }
+  const double mean = S / (double)n;
+  const double var = fmax(S2 / (double)n - mean * mean, 0.0);
+  shift[c] = (float)mean;
+  scale[c] = (float)(1.0 / fmax(sqrt(var), 1e-6));
+}
+
+// db[n] = sum_b dZ[b, n]: the tiles' column sums (tile_kernel's colsum) added in a fixed order, so that
+// training stays bitwise reproducible.
 __global__ void bias_grad_sum_kernel(const float* part, int chunks, int N, float* db) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
@@ -185,10 +345,9 @@ __global__ void bias_grad_sum_kernel(const float* part, int chunks, int N, float
   db[n] = s;
 }
 
-// Adam with bias correction (the update of design/optimize.py:95-104, applied to the MLP), and the
-// bf16 copy of the new weights.
+// Adam with bias correction (the update of design/optimize.py:95-104, applied to the MLP).
 __global__ void adam_kernel(float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2,
-                            float eps, float c1, float c2, uint16_t* wbf, long nbf) {
+                            float eps, float c1, float c2) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -196,16 +355,43 @@ __global__ void adam_kernel(float* w, const float* g, float* m, float* v, long n
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  const float wn = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  w[i] = wn;
-  if (wbf && i < nbf) wbf[i] = f2bf(wn);
+  w[i] = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-int gemm(const GemmArgs& g, hipStream_t s) {
-  if (g.M <= 0 || g.N <= 0) return BMC_OK;
-  dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM);
-  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, s, g);
+int tiles(const TileArgs& a, hipStream_t s) {
+  if (a.rows <= 0 || a.cols <= 0) return BMC_OK;
+  hipLaunchKernelGGL(tile_kernel, dim3((a.cols + 63) / 64, (a.rows + 63) / 64), dim3(256), 0, s, a);
   BMC_HIP_CHECK(hipGetLastError());
+  return BMC_OK;
+}
+
+// Launch shape: 128 x 128 tiles when they alone fill the CUs, else 64 x 64; K split until about 512
+// workgroups run, each keeping at least four slabs.
+int gemm(GemmArgs g, float* part, size_t part_floats, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return BMC_OK;
+  const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
+  const long t64 = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
+  const bool big = t128 >= 256;
+  const long tiles_n = big ? t128 : t64;
+  int splits = 1;
+  const int slabs = (g.K + BK - 1) / BK;
+  while (tiles_n * splits < 512 && slabs / (2 * splits) >= 4 &&
+         (size_t)(2 * splits) * g.M * g.N <= part_floats)
+    splits *= 2;
+  g.splits = splits;
+  g.kchunk = ((slabs + splits - 1) / splits) * BK;
+  g.part = part;
+  const dim3 grid(big ? (g.N + 127) / 128 : (g.N + 63) / 64, big ? (g.M + 127) / 128 : (g.M + 63) / 64, splits);
+  if (big)
+    hipLaunchKernelGGL((gemm_nt_kernel<128, 128>), grid, dim3(256), 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_nt_kernel<64, 64>), grid, dim3(256), 0, s, g);
+  BMC_HIP_CHECK(hipGetLastError());
+  if (splits > 1) {
+    const long n = (long)g.M * g.N;
+    hipLaunchKernelGGL(gemm_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g);
+    BMC_HIP_CHECK(hipGetLastError());
+  }
   return BMC_OK;
 }
 
@@ -219,16 +405,21 @@ struct bmc_mlp {
   int L = 0;                 // layers
   std::vector<int> dims;     // L + 1
   int max_batch = 0;
+  long ldb = 0;              // padded batch (leading dimension of every transposed activation)
   std::vector<long> woff, boff;  // offsets of W_l and b_l in the flat parameter vector
   long n_params = 0;
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // fp32 [n_params]
-  uint16_t* wbf = nullptr;                                        // bf16 copy [n_params]
-  // activations: per layer output pre-activation and post (fp32 + bf16), input bf16
+  std::vector<uint16_t*> wbf, wbfT;  // W_l as bf16 [out, pad8(in)] and [in, pad8(out)]
+  // per layer l (output width dims[l+1]): pre-activation, fp32 output (last layer), gradient of the
+  // output / pre-activation (fp32), bf16 copies in both orientations
   std::vector<float*> pre, act, dact;
-  std::vector<uint16_t*> act_bf, dact_bf;
-  uint16_t* x_bf = nullptr;
+  std::vector<uint16_t*> act_bf, act_bfT, dz_bf, dz_bfT;
+  uint16_t *x_bf = nullptr, *x_bfT = nullptr;  // standardised input [B, pad8(d0)] and [d0, ldb]
+  float *shift = nullptr, *scale = nullptr;    // input standardisation [d0] (null until set)
   float* loss = nullptr;
-  float* bias_part = nullptr;  // [ceil(max_batch / kBiasRows), max width] partial bias gradients
+  float* bias_part = nullptr;  // [ceil(max_batch / 64), max width] partial bias gradients (tile column sums)
+  float* part = nullptr;       // split-K partials
+  size_t part_floats = 0;
   int step = 0;
 };
 
@@ -243,43 +434,82 @@ void mlp_free(bmc_mlp* h) {
   F(h->g);
   F(h->m);
   F(h->v);
-  F(h->wbf);
+  for (auto p : h->wbf) F(p);
+  for (auto p : h->wbfT) F(p);
   for (auto p : h->pre) F(p);
   for (auto p : h->act) F(p);
   for (auto p : h->dact) F(p);
   for (auto p : h->act_bf) F(p);
-  for (auto p : h->dact_bf) F(p);
+  for (auto p : h->act_bfT) F(p);
+  for (auto p : h->dz_bf) F(p);
+  for (auto p : h->dz_bfT) F(p);
   F(h->x_bf);
+  F(h->x_bfT);
+  F(h->shift);
+  F(h->scale);
   F(h->loss);
   F(h->bias_part);
+  F(h->part);
   delete h;
 }
 
-int forward(bmc_mlp* h, const float* X, int B, hipStream_t s) {
-  const long nin = (long)B * h->dims[0];
-  hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((nin + 255) / 256), dim3(256), 0, s, X, h->x_bf, nin);
-  BMC_HIP_CHECK(hipGetLastError());
+// bf16 copies of every W_l in both orientations, from the fp32 master.
+int pack_weights(bmc_mlp* h, hipStream_t s) {
   for (int l = 0; l < h->L; l++) {
     const int K = h->dims[l], N = h->dims[l + 1];
+    TileArgs t{};
+    t.mode = TILE_INPUT;
+    t.rows = N;
+    t.cols = K;
+    t.x = h->w + h->woff[l];
+    t.ldx = K;
+    t.bf = h->wbf[l];
+    t.ldo = pad8(K);
+    t.bfT = h->wbfT[l];
+    t.ldt = pad8(N);
+    int rc = tiles(t, s);
+    if (rc) return rc;
+  }
+  return BMC_OK;
+}
+
+int forward(bmc_mlp* h, const float* X, int B, hipStream_t s) {
+  TileArgs t{};
+  t.mode = TILE_INPUT;
+  t.rows = B;
+  t.cols = h->dims[0];
+  t.x = X;
+  t.ldx = h->dims[0];
+  t.aux = h->shift;
+  t.scale = h->scale;
+  t.bf = h->x_bf;
+  t.ldo = pad8(h->dims[0]);
+  t.bfT = h->x_bfT;
+  t.ldt = h->ldb;
+  int rc = tiles(t, s);
+  if (rc) return rc;
+  for (int l = 0; l < h->L; l++) {
+    const int K = h->dims[l], N = h->dims[l + 1];
+    const bool last = l + 1 == h->L;
     GemmArgs g{};
     g.M = B;
     g.N = N;
     g.K = K;
     g.A = l == 0 ? h->x_bf : h->act_bf[l - 1];
-    g.sam = K;
-    g.sak = 1;
-    g.B = h->wbf + h->woff[l];  // W [N, K]: B[k, n] = W[n, k]
-    g.sbk = 1;
-    g.sbn = K;
-    g.C = h->act[l];
-    g.ldc = N;
-    g.bias = h->w + h->boff[l];
-    g.act = (l + 1 < h->L) ? 1 : 0;
-    g.Cpre = h->pre[l];
-    g.Cbf = h->act_bf[l];
-    g.alpha = 1.f;
-    int rc = gemm(g, s);
-    if (rc) return rc;
+    g.lda = pad8(K);
+    g.B = h->wbf[l];
+    g.ldb = pad8(K);
+    g.e.alpha = 1.f;
+    g.e.bias = h->w + h->boff[l];
+    g.e.act = last ? 0 : 1;
+    g.e.ldc = N;
+    g.e.C = last ? h->act[l] : nullptr;  // hidden layers need only the pre-activation and the bf16 copies
+    g.e.Cpre = last ? nullptr : h->pre[l];
+    g.e.Cbf = last ? nullptr : h->act_bf[l];
+    g.e.ldcb = pad8(N);
+    g.e.CbfT = last ? nullptr : h->act_bfT[l];
+    g.e.ldct = h->ldb;
+    if ((rc = gemm(g, h->part, h->part_floats, s))) return rc;
   }
   return BMC_OK;
 }
@@ -287,60 +517,56 @@ int forward(bmc_mlp* h, const float* X, int B, hipStream_t s) {
 // Gradients of the MSE loss w.r.t. every parameter into h->g (after forward at the same batch).
 int backward(bmc_mlp* h, const float* T, int B, hipStream_t s) {
   const int L = h->L;
-  const int D = h->dims[L];
-  const long nout = (long)B * D;
-  hipLaunchKernelGGL(mse_grad_kernel, dim3((nout + 255) / 256), dim3(256), 0, s, h->act[L - 1], T, B, D,
-                     h->dact[L - 1], h->dact_bf[L - 1], h->loss);
-  BMC_HIP_CHECK(hipGetLastError());
   for (int l = L - 1; l >= 0; l--) {
     const int K = h->dims[l], N = h->dims[l + 1];
-    // dZ_l: for hidden layers dact * gelu'(pre); the last layer is linear
-    float* dZ = h->dact[l];
-    uint16_t* dZbf = h->dact_bf[l];
-    if (l + 1 < L) {
-      const long n = (long)B * N;
-      hipLaunchKernelGGL(gelu_back_kernel, dim3((n + 255) / 256), dim3(256), 0, s, h->dact[l], h->pre[l], n, dZ,
-                         dZbf);
-      BMC_HIP_CHECK(hipGetLastError());
-    }
-    // dW_l [N, K] = dZ^T [N, B] X_l [B, K]
+    // dZ_l: 2 (Y - T) / (B D) for the linear last layer, dact * gelu'(pre) for hidden layers; fp32 in
+    // dact[l], bf16 in both orientations
+    TileArgs t{};
+    t.mode = l + 1 == L ? TILE_MSE : TILE_GELU;
+    t.rows = B;
+    t.cols = N;
+    t.x = l + 1 == L ? h->act[l] : h->dact[l];
+    t.ldx = N;
+    t.aux = l + 1 == L ? T : h->pre[l];
+    t.out32 = h->dact[l];
+    t.bf = h->dz_bf[l];
+    t.ldo = pad8(N);
+    t.bfT = h->dz_bfT[l];
+    t.ldt = h->ldb;
+    t.loss = h->loss;
+    t.colsum = h->bias_part;
+    int rc = tiles(t, s);
+    if (rc) return rc;
+    // dW_l [N, K] = dZ^T [N, B] X_l [B, K]: both operands batch-contiguous
     GemmArgs gw{};
     gw.M = N;
     gw.N = K;
     gw.K = B;
-    gw.A = dZbf;  // A[n, b] = dZ[b, n]
-    gw.sam = 1;
-    gw.sak = N;
-    gw.B = l == 0 ? h->x_bf : h->act_bf[l - 1];  // B[b, k] = X[b, k]
-    gw.sbk = K;
-    gw.sbn = 1;
-    gw.C = h->g + h->woff[l];
-    gw.ldc = K;
-    gw.alpha = 1.f;
-    int rc = gemm(gw, s);
-    if (rc) return rc;
-    const int chunks = (B + kBiasRows - 1) / kBiasRows;
-    hipLaunchKernelGGL(bias_grad_partial_kernel, dim3((N + 63) / 64, chunks), dim3(256), 0, s, dZ, B, N,
-                       h->bias_part);
-    hipLaunchKernelGGL(bias_grad_sum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, h->bias_part, chunks, N,
+    gw.A = h->dz_bfT[l];
+    gw.lda = h->ldb;
+    gw.B = l == 0 ? h->x_bfT : h->act_bfT[l - 1];
+    gw.ldb = h->ldb;
+    gw.e.C = h->g + h->woff[l];
+    gw.e.ldc = K;
+    gw.e.alpha = 1.f;
+    if ((rc = gemm(gw, h->part, h->part_floats, s))) return rc;
+    hipLaunchKernelGGL(bias_grad_sum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, h->bias_part, (B + 63) / 64, N,
                        h->g + h->boff[l]);
     BMC_HIP_CHECK(hipGetLastError());
     if (l > 0) {
-      // dX_l [B, K] = dZ [B, N] W [N, K], into dact[l - 1]
+      // dX_l [B, K] = dZ [B, N] W [N, K]: A = dZ (N-contiguous), B^T = W^T [K, N]
       GemmArgs gx{};
       gx.M = B;
       gx.N = K;
       gx.K = N;
-      gx.A = dZbf;
-      gx.sam = N;
-      gx.sak = 1;
-      gx.B = h->wbf + h->woff[l];  // B[n, k] = W[n, k]
-      gx.sbk = K;
-      gx.sbn = 1;
-      gx.C = h->dact[l - 1];
-      gx.ldc = K;
-      gx.alpha = 1.f;
-      if ((rc = gemm(gx, s))) return rc;
+      gx.A = h->dz_bf[l];
+      gx.lda = pad8(N);
+      gx.B = h->wbfT[l];
+      gx.ldb = pad8(N);
+      gx.e.C = h->dact[l - 1];
+      gx.e.ldc = K;
+      gx.e.alpha = 1.f;
+      if ((rc = gemm(gx, h->part, h->part_floats, s))) return rc;
     }
   }
   return BMC_OK;
@@ -359,6 +585,8 @@ int bmc_mlp_create(int n_layers, const int32_t* dims, int max_batch, uint64_t se
   h->L = n_layers;
   h->dims.assign(dims, dims + n_layers + 1);
   h->max_batch = max_batch;
+  h->ldb = pad8(max_batch);
+  long widest = 0, biggest_w = 0;
   for (int l = 0; l < n_layers; l++) {
     if (dims[l] < 1 || dims[l + 1] < 1) {
       delete h;
@@ -369,27 +597,36 @@ int bmc_mlp_create(int n_layers, const int32_t* dims, int max_batch, uint64_t se
     h->n_params += (long)dims[l] * dims[l + 1];
     h->boff.push_back(h->n_params);
     h->n_params += dims[l + 1];
+    widest = std::max<long>(widest, std::max(dims[l], dims[l + 1]));
+    biggest_w = std::max<long>(biggest_w, (long)dims[l] * dims[l + 1]);
   }
-  auto A = [&](void** p, size_t bytes) { return hipMalloc(p, bytes) == hipSuccess; };
+  // split-K partials: up to 16 splits of a weight gradient or 2 of a batch-sized product
+  h->part_floats = (size_t)std::max(16 * biggest_w, 2L * max_batch * widest);
+  auto A = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)) == hipSuccess; };
+  const long d0 = dims[0];
   bool ok = A((void**)&h->w, 4 * h->n_params) && A((void**)&h->g, 4 * h->n_params) &&
             A((void**)&h->m, 4 * h->n_params) && A((void**)&h->v, 4 * h->n_params) &&
-            A((void**)&h->wbf, 2 * h->n_params) && A((void**)&h->x_bf, 2L * max_batch * dims[0]) &&
-            A((void**)&h->loss, 4) &&
-            A((void**)&h->bias_part, 4L * ((max_batch + kBiasRows - 1) / kBiasRows) *
-                                         *std::max_element(dims + 1, dims + n_layers + 1));
+            A((void**)&h->x_bf, 2 * max_batch * pad8(d0)) && A((void**)&h->x_bfT, 2 * d0 * h->ldb) &&
+            A((void**)&h->loss, 4) && A((void**)&h->part, 4 * h->part_floats) &&
+            A((void**)&h->bias_part, 4L * ((max_batch + 63) / 64) * *std::max_element(dims + 1, dims + n_layers + 1));
   for (int l = 0; ok && l < n_layers; l++) {
-    const size_t n = (size_t)max_batch * dims[l + 1];
-    float *a, *b, *c;
-    uint16_t *d, *e;
-    ok = A((void**)&a, 4 * n) && A((void**)&b, 4 * n) && A((void**)&c, 4 * n) && A((void**)&d, 2 * n) &&
-         A((void**)&e, 2 * n);
-    if (ok) {
-      h->pre.push_back(a);
-      h->act.push_back(b);
-      h->dact.push_back(c);
-      h->act_bf.push_back(d);
-      h->dact_bf.push_back(e);
-    }
+    const long K = dims[l], N = dims[l + 1];
+    const size_t n = (size_t)max_batch * N;
+    float *pre = nullptr, *act = nullptr, *dact = nullptr;
+    uint16_t *abf = nullptr, *abfT = nullptr, *zbf = nullptr, *zbfT = nullptr, *wb = nullptr, *wbT = nullptr;
+    ok = A((void**)&pre, 4 * n) && A((void**)&act, 4 * n) && A((void**)&dact, 4 * n) &&
+         A((void**)&abf, 2 * max_batch * pad8(N)) && A((void**)&abfT, 2 * N * h->ldb) &&
+         A((void**)&zbf, 2 * max_batch * pad8(N)) && A((void**)&zbfT, 2 * N * h->ldb) &&
+         A((void**)&wb, 2 * N * pad8(K)) && A((void**)&wbT, 2 * K * pad8(N));
+    h->pre.push_back(pre);
+    h->act.push_back(act);
+    h->dact.push_back(dact);
+    h->act_bf.push_back(abf);
+    h->act_bfT.push_back(abfT);
+    h->dz_bf.push_back(zbf);
+    h->dz_bfT.push_back(zbfT);
+    h->wbf.push_back(wb);
+    h->wbfT.push_back(wbT);
   }
   if (!ok) {
     mlp_free(h);
@@ -431,8 +668,8 @@ int bmc_mlp_set_params(bmc_mlp* h, const float* host_params) {
     return BMC_EINVAL;
   }
   BMC_HIP_CHECK(hipMemcpy(h->w, host_params, 4 * h->n_params, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((h->n_params + 255) / 256), dim3(256), 0, 0, h->w, h->wbf, h->n_params);
-  BMC_HIP_CHECK(hipGetLastError());
+  int rc = pack_weights(h, 0);
+  if (rc) return rc;
   BMC_HIP_CHECK(hipDeviceSynchronize());
   h->step = 0;
   return BMC_OK;
@@ -447,11 +684,83 @@ int bmc_mlp_get_params(const bmc_mlp* h, float* host_params) {
   return BMC_OK;
 }
 
+int bmc_mlp_set_input_norm(bmc_mlp* h, const float* shift_host, const float* scale_host) {
+  if (!h || (!shift_host) != (!scale_host)) {
+    set_error("bmc_mlp_set_input_norm: give both shift and scale, or neither");
+    return BMC_EINVAL;
+  }
+  const size_t bytes = 4 * (size_t)h->dims[0];
+  if (!shift_host) {
+    if (h->shift) (void)hipFree(h->shift);
+    if (h->scale) (void)hipFree(h->scale);
+    h->shift = h->scale = nullptr;
+    return BMC_OK;
+  }
+  if (!h->shift && (hipMalloc(&h->shift, bytes) != hipSuccess || hipMalloc(&h->scale, bytes) != hipSuccess)) {
+    set_error("bmc_mlp_set_input_norm: device allocation failed");
+    return BMC_ENOMEM;
+  }
+  BMC_HIP_CHECK(hipMemcpy(h->shift, shift_host, bytes, hipMemcpyHostToDevice));
+  BMC_HIP_CHECK(hipMemcpy(h->scale, scale_host, bytes, hipMemcpyHostToDevice));
+  return BMC_OK;
+}
+
+int bmc_mlp_get_input_norm(const bmc_mlp* h, float* shift_host, float* scale_host) {
+  if (!h || !shift_host || !scale_host) {
+    set_error("bmc_mlp_get_input_norm: null argument");
+    return BMC_EINVAL;
+  }
+  const size_t bytes = 4 * (size_t)h->dims[0];
+  if (!h->shift) {
+    for (int i = 0; i < h->dims[0]; i++) {
+      shift_host[i] = 0.f;
+      scale_host[i] = 1.f;
+    }
+    return BMC_OK;
+  }
+  BMC_HIP_CHECK(hipMemcpy(shift_host, h->shift, bytes, hipMemcpyDeviceToHost));
+  BMC_HIP_CHECK(hipMemcpy(scale_host, h->scale, bytes, hipMemcpyDeviceToHost));
+  return BMC_OK;
+}
+
+int bmc_mlp_fit_input_norm(bmc_mlp* h, const float* x_dev, int64_t n, void* stream) {
+  if (!h || !x_dev || n < 2) {
+    set_error("bmc_mlp_fit_input_norm: need at least two rows");
+    return BMC_EINVAL;
+  }
+  const int d = h->dims[0];
+  const size_t bytes = 4 * (size_t)d;
+  if (!h->shift && (hipMalloc(&h->shift, bytes) != hipSuccess || hipMalloc(&h->scale, bytes) != hipSuccess)) {
+    set_error("bmc_mlp_fit_input_norm: device allocation failed");
+    return BMC_ENOMEM;
+  }
+  const long chunks = (n + kStatRows - 1) / kStatRows;
+  double* part = nullptr;
+  if (hipMalloc(&part, sizeof(double) * 2 * chunks * d) != hipSuccess) {
+    set_error("bmc_mlp_fit_input_norm: device allocation failed");
+    return BMC_ENOMEM;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(column_stats_partial_kernel, dim3((d + 63) / 64, (unsigned)chunks), dim3(256), 0, s, x_dev,
+                     (long)n, d, part);
+  hipLaunchKernelGGL(column_stats_final_kernel, dim3((d + 255) / 256), dim3(256), 0, s, part, (int)chunks, (long)n, d,
+                     h->shift, h->scale);
+  hipError_t e = hipGetLastError();
+  hipError_t se = hipStreamSynchronize(s);
+  (void)hipFree(part);
+  if (e != hipSuccess || se != hipSuccess) {
+    set_error(std::string("bmc_mlp_fit_input_norm: ") + hipGetErrorString(e != hipSuccess ? e : se));
+    return BMC_EHIP;
+  }
+  return BMC_OK;
+}
+
 int bmc_mlp_forward(bmc_mlp* h, const float* x_dev, int batch, float* y_dev, void* stream) {
   if (!h || !x_dev || !y_dev || batch < 0 || batch > h->max_batch) {
     set_error("bmc_mlp_forward: bad arguments (batch above max_batch?)");
     return BMC_EINVAL;
   }
+  if (batch == 0) return BMC_OK;
   hipStream_t s = (hipStream_t)stream;
   int rc = forward(h, x_dev, batch, s);
   if (rc) return rc;
@@ -498,9 +807,9 @@ int bmc_mlp_train_step(bmc_mlp* h, const float* x_dev, const float* t_dev, int b
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - powf(b1, (float)h->step), c2 = 1.f - powf(b2, (float)h->step);
   hipLaunchKernelGGL(adam_kernel, dim3((h->n_params + 255) / 256), dim3(256), 0, s, h->w, h->g, h->m, h->v,
-                     h->n_params, lr, b1, b2, 1e-8f, c1, c2, h->wbf, h->n_params);
+                     h->n_params, lr, b1, b2, 1e-8f, c1, c2);
   BMC_HIP_CHECK(hipGetLastError());
-  return BMC_OK;
+  return pack_weights(h, s);
 }
 
 }  // extern "C"

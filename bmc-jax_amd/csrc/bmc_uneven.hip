@@ -322,6 +322,14 @@ __global__ void __launch_bounds__(64) u_normal_eq_kernel(ULm a) {
     c = fma(r, r, c);
   }
   c = u_wave_sum(c);
+  if (a.early && sqrt(c / (double)R) <= a.rms_tol) {
+    // already within the tolerance where its Jacobian was just taken: no step, stopped
+    if (lane == 0) {
+      a.cost[v] = c;
+      a.active[v] = 0;
+    }
+    return;
+  }
   for (int p = 0; p < P; p++) {
     double acc = 0.0;
     for (int o = lane; o < R; o += 64) {
@@ -786,10 +794,16 @@ extern "C" int bmc_ubatch_lm_fit(bmc_ubatch* b, const void* z_meas_dev, double* 
   hipStream_t st = (hipStream_t)stream;
   const bool use_graph = (flags & BMC_LM_GRAPH) != 0;
   hipStream_t own = nullptr;
-  if (use_graph && st == nullptr) {
-    BMC_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+  if (use_graph && st == nullptr) {  // the null stream cannot be captured
+    if ((rc = stream_after_null(&own)) != BMC_OK) return rc;
     st = own;
   }
+  struct OwnGuard {
+    hipStream_t s;
+    ~OwnGuard() {
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } own_guard{own};
   hipLaunchKernelGGL(u_init_kernel, dim3((b->n_prob + 255) / 256), dim3(256), 0, st, L, lam0);
   BMC_HIP_CHECK(hipGetLastError());
   auto iteration = [&]() -> int {
@@ -833,6 +847,5 @@ extern "C" int bmc_ubatch_lm_fit(bmc_ubatch* b, const void* z_meas_dev, double* 
     BMC_HIP_CHECK(hipMemcpyAsync(cost_dev, cost, b->n_prob * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
   BMC_HIP_CHECK(hipStreamSynchronize(st));
-  if (own) (void)hipStreamDestroy(own);
   return result;
 }

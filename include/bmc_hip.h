@@ -163,7 +163,9 @@ int bmc_simulate_trajectory(const bmc_plan* plan, int n_voxels, int n_cest, int 
  *   cost_traj_dev[V, n_iter] float64 or NULL cost after every iteration (the reference's C_traj)
  *   x_traj_dev   [V, n_iter, P] float64 or NULL
  *   flags        BMC_LM_EARLY_EXIT: stop a problem once it has converged (never the reference's
- *                behaviour, which runs every iteration); BMC_LM_GRAPH: replay the iteration as a
+ *                behaviour, which runs every iteration): its rms at most early_rms (checked after
+ *                every accepted step and, from the Jacobian sweep's own state, before the first step),
+ *                a relative cost decrease of at most early_ftol, or lam above 1e12; BMC_LM_GRAPH: replay the iteration as a
  *                captured hipGraph; BMC_LM_START_CHECK (with BMC_LM_EARLY_EXIT): simulate every start
  *                first and stop, before any Jacobian, the problems whose start rms is at most
  *                early_rms (they return x0 after 0 iterations); BMC_LM_FORWARD_JACOBIAN: Jacobian sweeps
@@ -280,6 +282,15 @@ int bmc_mlp_gradients(bmc_mlp* mlp, const float* x_dev, const float* t_dev, int 
                       float* loss_host, void* stream);
 int bmc_mlp_train_step(bmc_mlp* mlp, const float* x_dev, const float* t_dev, int batch, float lr, float* loss_dev,
                        void* stream);
+/* Input standardisation applied before the first layer, x -> (x - shift) * scale per input feature
+ * (without one, the identity). bmc_mlp_fit_input_norm sets shift to the column means and scale to the
+ * inverse column standard deviations of x_dev [n, dims[0]] (float64 sums in a fixed order; a standard
+ * deviation below 1e-6 counts as 1e-6); set / get move both vectors [dims[0]] from / to the host (set
+ * with both NULL removes it). A measured Z-spectrum varies by a few percent around a mean near 1, below
+ * BF16's resolution there: standardised, the features use the GEMM's BF16 operands fully. */
+int bmc_mlp_fit_input_norm(bmc_mlp* mlp, const float* x_dev, int64_t n, void* stream);
+int bmc_mlp_set_input_norm(bmc_mlp* mlp, const float* shift_host, const float* scale_host);
+int bmc_mlp_get_input_norm(const bmc_mlp* mlp, float* shift_host, float* scale_host);
 
 #ifdef __cplusplus
 }

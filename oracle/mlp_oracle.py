@@ -35,9 +35,14 @@ def unflatten(w, dims):
     return Ws, bs
 
 
-def forward(w, dims, X):
+def forward(w, dims, X, shift=None, scale=None):
+    """Output and the saved activations; the input is standardised first, (X - shift) * scale in float32
+    as the kernel does, when shift / scale are given."""
     Ws, bs = unflatten(w, dims)
-    a = bf16(X)
+    Xs = np.asarray(X, np.float32)
+    if shift is not None:
+        Xs = (Xs - np.asarray(shift, np.float32)) * np.asarray(scale, np.float32)
+    a = bf16(Xs)
     pres, acts = [], [a]
     for l, (W, b) in enumerate(zip(Ws, bs)):
         z = a @ bf16(W).T + b
@@ -50,10 +55,10 @@ def forward(w, dims, X):
     return a, pres, acts
 
 
-def gradients(w, dims, X, T):
+def gradients(w, dims, X, T, shift=None, scale=None):
     """Flat gradient of mean((Y - T)^2) and the loss."""
     Ws, bs = unflatten(w, dims)
-    Y, pres, acts = forward(w, dims, X)
+    Y, pres, acts = forward(w, dims, X, shift, scale)
     n = Y.size
     loss = float(np.sum((Y - T) ** 2) / n)
     dA = 2.0 * (Y - T) / n

@@ -44,14 +44,20 @@ def test_the_proposal_network_at_its_size(torch_mod, batch):
     dims = (244, 512, 512, 256, 5)
     mlp = MLP(dims=dims, max_batch=4096, seed=11)
     rng = np.random.default_rng(2)
-    X = rng.uniform(0.0, 1.0, size=(batch, 244)).astype(np.float32)
+    # Z-spectrum-like inputs: a mean near 1 per feature with a few percent of spread, standardised
+    X = (0.9 - 0.3 * np.sin(np.linspace(0, 3, 244)) + 0.03 * rng.normal(size=(batch, 244))).astype(np.float32)
     T = rng.normal(size=(batch, 5)).astype(np.float32)
+    Xd = torch.as_tensor(X, device="cuda")
+    mlp.fit_input_norm(Xd)
+    shift, scale = mlp.input_norm()
+    np.testing.assert_allclose(shift, X.astype(np.float64).mean(axis=0), rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(1.0 / scale, X.astype(np.float64).std(axis=0), rtol=1e-5)
     w = mlp.get_params().astype(np.float64)
-    Y = mlp.forward(torch.as_tensor(X, device="cuda")).cpu().numpy()
-    Yo, _, _ = MO.forward(w, dims, X)
+    Y = mlp.forward(Xd).cpu().numpy()
+    Yo, _, _ = MO.forward(w, dims, X, shift, scale)
     assert np.max(np.abs(Y - Yo)) < 2e-3 * max(1.0, np.abs(Yo).max())
-    g, loss = mlp.gradients(torch.as_tensor(X, device="cuda"), torch.as_tensor(T, device="cuda"))
-    go, lo = MO.gradients(w, dims, X, T)
+    g, loss = mlp.gradients(Xd, torch.as_tensor(T, device="cuda"))
+    go, lo = MO.gradients(w, dims, X, T, shift, scale)
     assert loss == pytest.approx(lo, rel=1e-3)
     off = 0
     for l in range(len(dims) - 1):

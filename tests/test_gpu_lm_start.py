@@ -1,6 +1,7 @@
 """bmc_lm_fit's start check (BMC_LM_START_CHECK with early exit): problems whose start already meets the
 rms tolerance stop before their first Jacobian and keep their start; the others take exactly the steps
-of a run without the check. Through the C ABI, float64, configuration 1's CW spectrum."""
+of a run without the check. Without the check, early exit stops a converged start on its first Jacobian
+sweep's state, before any step. Through the C ABI, float64, configuration 1's CW spectrum."""
 import numpy as np
 import pytest
 
@@ -41,7 +42,9 @@ def test_start_check_stops_converged_starts_only():
     np.testing.assert_array_equal(x[0::2], x0[0::2])
     assert np.all(ct[0::2] < 1e-24)
     xp, ctp, ndp = runs["plain"]
-    assert np.all(ndp[0::2] == 1)               # without the check a converged start still takes a step
+    # without the separate sweep the first Jacobian sweep's own state stops a converged start: no step
+    assert np.all(ndp[0::2] == 0)
+    np.testing.assert_array_equal(xp[0::2], x0[0::2])
     np.testing.assert_array_equal(nd[1::2], ndp[1::2])
     np.testing.assert_array_equal(x[1::2], xp[1::2])
     np.testing.assert_allclose(x[1::2], np.tile(truth, (V // 2, 1)), rtol=1e-5)   # rms 1e-9 leaves ~1e-6
