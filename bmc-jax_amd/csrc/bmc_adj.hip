@@ -40,8 +40,9 @@ struct AdjArgs {
   int V, dtype, stride;
   const void* params;  // [V, stride]
   const void* m0;      // [V, N]
-  const void* dm0;     // [V, P, N] or null
-  int P;
+  const void* dm0;     // [V, p_stride, N] or null
+  int P;               // tangents of this launch: entries p_off .. p_off + P - 1 of the caller's p_stride
+  int p_stride, p_off;
   int t_kind[kMaxTan], t_idx[kMaxTan];
   double t_scale[kMaxTan];
   int jc;
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(256, BMC_ADJ_WAVES_PER_SIMD) adj_kernel(AdjArg
     if (p < 0)
       val = load_real(a.m0, (long)v * N + r, a.dtype);
     else if (p < P && a.dm0)
-      val = load_real(a.dm0, ((long)v * a.P + p) * N + r, a.dtype);
+      val = load_real(a.dm0, ((long)v * a.p_stride + a.p_off + p) * N + r, a.dtype);
     st0[i] = val;
   }
   __syncthreads();
@@ -356,7 +357,7 @@ __global__ void __launch_bounds__(256, BMC_ADJ_WAVES_PER_SIMD) adj_kernel(AdjArg
       lam[i] = (i == a.jc && a.gz && !dead) ? a.gz[vo] : 0.0;
 #pragma unroll
       for (int p = 0; p < PA; p++)
-        mu[p][i] = (p < P && i == a.jc && a.gj && !dead) ? a.gj[vo * a.P + p] * a.t_scale[p] : 0.0;
+        mu[p][i] = (p < P && i == a.jc && a.gj && !dead) ? a.gj[vo * a.p_stride + a.p_off + p] * a.t_scale[p] : 0.0;
     }
   }
   bool live = !dead;
@@ -499,7 +500,8 @@ __global__ void __launch_bounds__(256, BMC_ADJ_WAVES_PER_SIMD) adj_kernel(AdjArg
 template <int NC, int P>
 static int launch_adj_np(const AdjArgs& a, hipStream_t stream) {
   constexpr int NA = 3 * (1 + NC) + 2;
-  constexpr int TEAM = NA <= 8 ? 8 : 16;  // one lane per propagator column (two teams per DPP row at 8)
+  // one lane per propagator column (two teams per DPP row at 8, a DPP row at 16, two rows beyond)
+  constexpr int TEAM = NA <= 8 ? 8 : (NA <= 16 ? 16 : 32);
   constexpr int TPB = 256 / TEAM;
   const long teams = (long)a.V * a.O;
   constexpr long kMaxBlocks = 1L << 23;
@@ -535,7 +537,13 @@ static void adj_segments(int T, int* K, int* n_seg) {
   *n_seg = (T + k - 1) / k;
 }
 
-static long adj_record(int nc, int P) { return (long)(1 + P) * (3 * (1 + nc) + 1) + 1; }
+// Tangents one launch carries (the rest run as further launches over the same workspace).
+constexpr int kAdjGroup = 2;
+
+static long adj_record(int nc, int P) {
+  const int Pg = P < kAdjGroup ? P : kAdjGroup;
+  return (long)(1 + Pg) * (3 * (1 + nc) + 1) + 1;
+}
 
 static long adj_teams_padded(int V, int O) {
   const long teams = (long)V * O;
@@ -569,14 +577,6 @@ extern "C" int bmc_simulate_adjoint(const bmc_plan* plan, int n_voxels, int n_ce
   int rc = fill_sim_args(plan, n_voxels, n_cest, dtype, params_dev, m0_dev, dm0_dev, n_tangents, tangent_slot,
                          tangent_scale, jac_component, s);
   if (rc != BMC_OK) return rc;
-  if (n_cest > 2) {
-    set_error("bmc_simulate_adjoint: at most 2 CEST pools");
-    return BMC_ENOTSUP;
-  }
-  if (n_tangents > 2) {
-    set_error("bmc_simulate_adjoint: at most 2 tangents");
-    return BMC_ENOTSUP;
-  }
   for (int p = 0; p < n_tangents; p++) {
     const int k = s.t_kind[p];
     if (k == TK_MT_DW || k == TK_MT_R2 || k == TK_B0 || k == TK_GAMMA || k == TK_B0INH || k == TK_RELB1) {
@@ -609,14 +609,8 @@ extern "C" int bmc_simulate_adjoint(const bmc_plan* plan, int n_voxels, int n_ce
   a.params = s.params;
   a.m0 = s.m0;
   a.dm0 = s.dm0;
-  a.P = n_tangents;
-  for (int p = 0; p < n_tangents; p++) {
-    a.t_kind[p] = s.t_kind[p];
-    a.t_idx[p] = s.t_idx[p];
-    a.t_scale[p] = s.t_scale[p];
-  }
+  a.p_stride = n_tangents;
   a.jc = jac_component;
-  a.gz = z_cot_dev;
   a.gj = jac_cot_dev;
   a.grad_amp = grad_amp_dev;
   a.grad_freq = grad_freq_dev;
@@ -624,12 +618,27 @@ extern "C" int bmc_simulate_adjoint(const bmc_plan* plan, int n_voxels, int n_ce
   adj_segments(a.T, &a.K, &a.n_seg);
   a.nan_l1 = s.nan_l1;
   hipStream_t st = (hipStream_t)stream;
-  switch (n_cest) {
-    case 0:
-      return launch_adj_nc<0>(a, st);
-    case 1:
-      return launch_adj_nc<1>(a, st);
-    default:
-      return launch_adj_nc<2>(a, st);
+  // L = gz.Z + sum_p gj_p.J_p is linear in the cotangents, so its gradient is the sum of the gradients of
+  // tangent groups of at most kAdjGroup (gz taken once, with the first group) added into the outputs
+  for (int p0 = 0; p0 == 0 || p0 < n_tangents; p0 += kAdjGroup) {
+    AdjArgs g = a;
+    g.P = std::min(kAdjGroup, n_tangents - p0);
+    g.p_off = p0;
+    g.gz = p0 == 0 ? z_cot_dev : nullptr;
+    for (int p = 0; p < g.P; p++) {
+      g.t_kind[p] = s.t_kind[p0 + p];
+      g.t_idx[p] = s.t_idx[p0 + p];
+      g.t_scale[p] = s.t_scale[p0 + p];
+    }
+    switch (n_cest) {
+      case 0: rc = launch_adj_nc<0>(g, st); break;
+      case 1: rc = launch_adj_nc<1>(g, st); break;
+      case 2: rc = launch_adj_nc<2>(g, st); break;
+      case 3: rc = launch_adj_nc<3>(g, st); break;
+      default: rc = launch_adj_nc<4>(g, st); break;
+    }
+    if (rc != BMC_OK) return rc;
+    if (n_tangents == 0) break;
   }
+  return BMC_OK;
 }

@@ -244,9 +244,9 @@ struct TileArgs {
   float* colsum;      // [ceil(rows / 64), cols] or null
 };
 
-__global__ void __launch_bounds__(256) tile_kernel(TileArgs a) {
+__device__ __forceinline__ void tile_body(const TileArgs& a, int tile_r, int tile_c) {
   __shared__ uint16_t t[64][66];
-  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int r0 = tile_r * 64, c0 = tile_c * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   float lsum = 0.f, csum = 0.f;
   const float inv = 1.0f / ((float)a.rows * (float)a.cols);
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) tile_kernel(TileArgs a) {
     cs[ty][tx] = csum;
     __syncthreads();
     if (ty == 0 && c0 + tx < a.cols)
-      a.colsum[(long)blockIdx.y * a.cols + c0 + tx] = (cs[0][tx] + cs[1][tx]) + (cs[2][tx] + cs[3][tx]);
+      a.colsum[(long)tile_r * a.cols + c0 + tx] = (cs[0][tx] + cs[1][tx]) + (cs[2][tx] + cs[3][tx]);
   }
   if (!a.bfT) return;
   __syncthreads();
@@ -293,6 +293,26 @@ __global__ void __launch_bounds__(256) tile_kernel(TileArgs a) {
     const int c = c0 + 4 * k + ty, r = r0 + tx;
     if (c < a.cols && r < a.rows) a.bfT[(long)c * a.ldt + r] = t[tx][4 * k + ty];
   }
+}
+
+__global__ void __launch_bounds__(256) tile_kernel(TileArgs a) { tile_body(a, blockIdx.y, blockIdx.x); }
+
+// Every layer's weight copies (bf16 W and W^T from the fp32 master) in one launch: block b takes tile
+// b - first[l] of the layer l whose range holds it.
+constexpr int kMaxPackLayers = 8;
+struct PackArgs {
+  int L;
+  int first[kMaxPackLayers + 1];  // tile ranges, first[L] = total
+  TileArgs t[kMaxPackLayers];
+};
+
+__global__ void __launch_bounds__(256) pack_kernel(PackArgs p) {
+  int l = 0;
+  while (l + 1 < p.L && (int)blockIdx.x >= p.first[l + 1]) l++;
+  const TileArgs& a = p.t[l];
+  const int tiles_c = (a.cols + 63) / 64;
+  const int i = blockIdx.x - p.first[l];
+  tile_body(a, i / tiles_c, i % tiles_c);
 }
 
 // Column mean and standard deviation of x [n, d] in float64 (the input standardisation), in two passes
@@ -337,12 +357,19 @@ __global__ void column_stats_final_kernel(const double* part, int chunks, long n
 
 // db[n] = sum_b dZ[b, n]: the tiles' column sums (tile_kernel's colsum) added in a fixed order, so that
 // training stays bitwise reproducible.
-__global__ void bias_grad_sum_kernel(const float* part, int chunks, int N, float* db) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+__global__ void __launch_bounds__(256) bias_grad_sum_kernel(const float* part, int chunks, int N, float* db) {
+  // 64 columns per block, four lanes per column each adding every fourth chunk, then the four in order
+  const int n = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
   float s = 0.f;
-  for (int k = 0; k < chunks; k++) s += part[(long)k * N + n];
-  db[n] = s;
+  if (n < N)
+    for (int k = q; k < chunks; k += 4) s += part[(long)k * N + n];
+  __shared__ float sh[4][64];
+  sh[q][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (q == 0 && n < N) {
+    const int j = threadIdx.x & 63;
+    db[n] = (sh[0][j] + sh[1][j]) + (sh[2][j] + sh[3][j]);
+  }
 }
 
 // Adam with bias correction (the update of design/optimize.py:95-104, applied to the MLP).
@@ -365,8 +392,10 @@ int tiles(const TileArgs& a, hipStream_t s) {
   return BMC_OK;
 }
 
-// Launch shape: 128 x 128 tiles when they alone fill the CUs, else 64 x 64; K split until about 512
-// workgroups run, each keeping at least four slabs.
+// Launch shape: 128 x 128 tiles when they alone fill the CUs, else 64 x 64; a product with fewer tiles
+// than CUs (the weight gradients, K = batch) splits K until about 512 workgroups run, each keeping at
+// least four slabs. A product that already has a tile per CU is not split: its partial sums would cost
+// a reduction pass longer than the product itself.
 int gemm(GemmArgs g, float* part, size_t part_floats, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return BMC_OK;
   const long t128 = (long)((g.M + 127) / 128) * ((g.N + 127) / 128);
@@ -375,7 +404,7 @@ int gemm(GemmArgs g, float* part, size_t part_floats, hipStream_t s) {
   const long tiles_n = big ? t128 : t64;
   int splits = 1;
   const int slabs = (g.K + BK - 1) / BK;
-  while (tiles_n * splits < 512 && slabs / (2 * splits) >= 4 &&
+  while (tiles_n < 256 && tiles_n * splits < 512 && slabs / (2 * splits) >= 4 &&
          (size_t)(2 * splits) * g.M * g.N <= part_floats)
     splits *= 2;
   g.splits = splits;
@@ -453,22 +482,31 @@ void mlp_free(bmc_mlp* h) {
   delete h;
 }
 
-// bf16 copies of every W_l in both orientations, from the fp32 master.
+// bf16 copies of every W_l in both orientations, from the fp32 master (one launch; layers beyond the
+// table's size take launches of their own).
 int pack_weights(bmc_mlp* h, hipStream_t s) {
-  for (int l = 0; l < h->L; l++) {
-    const int K = h->dims[l], N = h->dims[l + 1];
-    TileArgs t{};
-    t.mode = TILE_INPUT;
-    t.rows = N;
-    t.cols = K;
-    t.x = h->w + h->woff[l];
-    t.ldx = K;
-    t.bf = h->wbf[l];
-    t.ldo = pad8(K);
-    t.bfT = h->wbfT[l];
-    t.ldt = pad8(N);
-    int rc = tiles(t, s);
-    if (rc) return rc;
+  for (int l0 = 0; l0 < h->L; l0 += kMaxPackLayers) {
+    PackArgs p{};
+    p.L = std::min(kMaxPackLayers, h->L - l0);
+    int tiles = 0;
+    for (int j = 0; j < p.L; j++) {
+      const int l = l0 + j, K = h->dims[l], N = h->dims[l + 1];
+      TileArgs& t = p.t[j];
+      t.mode = TILE_INPUT;
+      t.rows = N;
+      t.cols = K;
+      t.x = h->w + h->woff[l];
+      t.ldx = K;
+      t.bf = h->wbf[l];
+      t.ldo = pad8(K);
+      t.bfT = h->wbfT[l];
+      t.ldt = pad8(N);
+      p.first[j] = tiles;
+      tiles += ((N + 63) / 64) * ((K + 63) / 64);
+    }
+    p.first[p.L] = tiles;
+    hipLaunchKernelGGL(pack_kernel, dim3(tiles), dim3(256), 0, s, p);
+    BMC_HIP_CHECK(hipGetLastError());
   }
   return BMC_OK;
 }
@@ -550,7 +588,7 @@ int backward(bmc_mlp* h, const float* T, int B, hipStream_t s) {
     gw.e.ldc = K;
     gw.e.alpha = 1.f;
     if ((rc = gemm(gw, h->part, h->part_floats, s))) return rc;
-    hipLaunchKernelGGL(bias_grad_sum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, h->bias_part, (B + 63) / 64, N,
+    hipLaunchKernelGGL(bias_grad_sum_kernel, dim3((N + 63) / 64), dim3(256), 0, s, h->bias_part, (B + 63) / 64, N,
                        h->g + h->boff[l]);
     BMC_HIP_CHECK(hipGetLastError());
     if (l > 0) {

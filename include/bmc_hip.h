@@ -216,8 +216,10 @@ int bmc_lm_fit_noalloc(const bmc_plan* plan, int n_voxels, int n_cest, int dtype
  *   jac_cot_dev  [V, O, P] float64 or NULL   cotangent of J (J includes tangent_scale)
  *   grad_amp_dev [O, T] float64 or NULL      += dL / d rf_amp   (Hz)
  *   grad_freq_dev[O, T] float64 or NULL      += dL / d rf_freq  (Hz)
- * Supported: n_cest <= 2, at most 2 tangents, and only tangents whose generator derivative does not
- * depend on the RF (water, CEST and MT rates and fractions, CEST shifts); other slots return
+ * Supported: every pool count the library is compiled for and up to BMC_MAX_TANGENTS tangents (run as
+ * launches of two tangents each over the one workspace, their gradients added: L is linear in the
+ * cotangents), for tangents whose generator derivative does not depend on the RF (water, CEST and MT
+ * rates and fractions, CEST shifts); B0, gamma, b0_inhom, rel_b1 and the MT shift and T2 return
  * BMC_ENOTSUP. dm0_dev as in bmc_simulate.
  */
 int bmc_adjoint_workspace_size(const bmc_plan* plan, int n_voxels, int n_cest, int n_tangents, int64_t* bytes);

@@ -40,6 +40,20 @@ def water_only():
     return O.make_params(1.2, 0.06, pools=[], b0=3.0)
 
 
+def three_cest(k=60.0, f=2e-3):
+    return O.make_params(1.2, 0.06, pools=[dict(f=f, k=k, dw=3.5, t1=1.3, t2=0.1),
+                                           dict(f=4e-3, k=700.0, dw=2.0, t1=1.3, t2=0.05),
+                                           dict(f=1e-3, k=150.0, dw=-1.0, t1=1.2, t2=0.02)],
+                         mt=dict(f=0.06, k=20.0, dw=-2.5, t1=1.0, t2=1e-5, lineshape="lorentzian"), b0=3.0)
+
+
+def four_cest(k=60.0, f=2e-3):
+    return O.make_params(1.2, 0.06, pools=[dict(f=f, k=k, dw=3.5, t1=1.3, t2=0.1),
+                                           dict(f=4e-3, k=700.0, dw=2.0, t1=1.3, t2=0.05),
+                                           dict(f=1e-3, k=150.0, dw=-1.0, t1=1.2, t2=0.02),
+                                           dict(f=2e-3, k=2000.0, dw=1.2, t1=1.0, t2=0.01)], b0=3.0)
+
+
 def train(ppm, n_pulses=3, n_samples=12, recovery_s=0.3):
     ev = O.pulsed_train(ppm, b1_peak_uT=2.0, n_pulses=n_pulses, pulse_s=0.01, n_samples=n_samples, gap_s=0.005,
                         recovery_s=recovery_s, b0_T=3.0, dtype=np.float64)
@@ -54,7 +68,16 @@ def train(ppm, n_pulses=3, n_samples=12, recovery_s=0.3):
 def slots_for(nc, P):
     s = [OC.slot_index("cest_k", 0, nc), OC.slot_index("cest_f", 0, nc)] if nc else \
         [OC.slot_index("water_r1", None, nc), OC.slot_index("water_r2", None, nc)]
+    if P > 2:   # more tangents than one launch carries: water rates, then the last pool's k and f
+        s = s + [OC.slot_index("water_r2", None, nc), OC.slot_index("water_r1", None, nc)]
+        if nc > 1:
+            s += [OC.slot_index("cest_k", nc - 1, nc), OC.slot_index("cest_f", nc - 1, nc)]
     return s[:P]
+
+
+def scales_for(nc, P):
+    base = [100.0, 0.001] if nc else [1.0, 10.0]
+    return (base + [10.0, 1.0, 100.0, 0.001])[:P]
 
 
 def gpu_adjoint(nat, records, ev, slots, scales, gz, gj, dtype=np.float64, want_freq=True):
@@ -110,6 +133,24 @@ def shifted(ev, field, direction, eps):
     return e
 
 
+@pytest.mark.parametrize("nc,P", [(3, 2), (2, 4)])
+def test_more_pools_and_tangents_match_the_float64_oracle(nat, nc, P):
+    """Three CEST pools and MT (16-lane teams), and four tangents (two launches whose gradients add)."""
+    rng = np.random.default_rng(30 + nc + P)
+    mk = {2: two_cest, 3: three_cest}[nc]
+    recs = [mk(k=50.0, f=1.5e-3), mk(k=600.0, f=4e-3)]
+    ev = train([-2.0, 3.5], n_pulses=2, n_samples=8)
+    slots, scales = slots_for(nc, P), scales_for(nc, P)
+    V, O_ = len(recs), ev["dt"].shape[0]
+    gz, gj = rng.normal(size=(V, O_)), rng.normal(size=(V, O_, P))
+    ga, gf = gpu_adjoint(nat, recs, ev, slots, scales, gz, gj)
+    for field, g in (("rf_amp", ga), ("rf_freq", gf)):
+        d = rng.normal(size=ev["dt"].shape)
+        fd = richardson(lambda e: oracle_loss(recs, shifted(ev, field, d, e), slots, scales, gz, gj), 1e-2)
+        ad = float(np.sum(g * d))
+        assert abs(ad - fd) <= 2e-7 * np.sum(np.abs(g * d)) + 1e-12, (field, ad, fd)
+
+
 @pytest.mark.parametrize("lineshape", ["superlorentzian", "lorentzian"])
 def test_directional_derivative_matches_the_float64_oracle(nat, lineshape):
     rng = np.random.default_rng(3)
@@ -127,13 +168,13 @@ def test_directional_derivative_matches_the_float64_oracle(nat, lineshape):
         assert abs(ad - fd) <= 2e-7 * np.sum(np.abs(g * d)) + 1e-12, (field, ad, fd)
 
 
-@pytest.mark.parametrize("nc,P", [(0, 1), (1, 0), (1, 1), (1, 2), (2, 2)])
+@pytest.mark.parametrize("nc,P", [(0, 1), (1, 0), (1, 1), (1, 2), (2, 2), (1, 3), (2, 6), (3, 2), (4, 4)])
 def test_components_match_differences_of_the_kernel_forward(nat, nc, P):
     rng = np.random.default_rng(10 + 3 * nc + P)
-    mk = {0: water_only, 1: three_pool, 2: two_cest}[nc]
+    mk = {0: water_only, 1: three_pool, 2: two_cest, 3: three_cest, 4: four_cest}[nc]
     recs = [mk(), mk()] if nc == 0 else [mk(k=70.0, f=2e-3), mk(k=500.0, f=3e-3)]
     ev = train([-3.0, 3.5], n_pulses=2, n_samples=8)
-    slots, scales = slots_for(nc, P), [100.0, 0.001][:P] if nc else [1.0, 10.0][:P]
+    slots, scales = slots_for(nc, P), scales_for(nc, P)
     V, O_ = len(recs), ev["dt"].shape[0]
     gz, gj = rng.normal(size=(V, O_)), (rng.normal(size=(V, O_, P)) if P else None)
     ga, gf = gpu_adjoint(nat, recs, ev, slots, scales, gz, gj)
