@@ -419,26 +419,29 @@ PHANTOM_PAIRS = (("GAG 100 / 50 mg/mL", "GAG100_pH6.31", "GAG50_pH6.20", 2.0),
                  ("carnosine 7.44 / 6.59", "carnosine_pH7.44", "carnosine_pH6.59", 1.0))
 
 
-def phantom_experiment(kind):
+def phantom_experiment(kind, tubes="pool"):
     """The measured seven-tube phantom (2026-09-27, 2.8936 T) as the reference's tutorials 08 (QUESP, eight
     B1 rungs) and 09 (QUEST, six saturation times) fit it: the six tubes with a CEST pool, each with its
     measured water R1, B0 offset and relative B1, water R2 from the PBS tube, start magnetisation 0.557, the
     scanner sequences converted to events (float32, reference row dropped), offsets within 1 ppm of water
-    left out of both data and model, the rungs stacked. Returns (experiment, spec, Z, noise, names, n_per_block)."""
+    left out of both data and model, the rungs stacked. tubes="all" adds the PBS tube (water only: a pool of
+    size and rate 0). Returns (experiment, spec, Z, noise, names, n_per_block)."""
     from bmc_jax.fitting import Experiment, FitSpec, FreeParam
     from bmc_jax.sequences import load_events
     from bmc_jax.simulation.params import SimParams, initial_state
     with np.load(os.path.join(PHANTOM_DIR, "phantom_7tubes.npz"), allow_pickle=False) as d:
         data = {k: d[k] for k in d.files}
-    tubes = np.flatnonzero(np.isfinite(data["vial_dw"]))
+    has_pool = np.isfinite(data["vial_dw"])
+    tubes = np.flatnonzero(has_pool) if tubes == "pool" else np.arange(has_pool.size)
+    pool = has_pool[tubes]
     ppm = data["ppm"]
     keep = np.abs(ppm) >= PHANTOM_EXCLUDE_PPM
     n = tubes.size
     one = np.ones(n, np.float32)
     f32 = lambda v: np.asarray(v, np.float32)
     sp = SimParams(water_r1=f32(data["R1"][tubes]), water_r2=f32(float(data["water_r2"]) * one), water_f=f32(one),
-                   cest_k=f32(np.full((n, 1), 900.0)), cest_f=f32(np.full((n, 1), 0.008)),
-                   cest_dw=f32(data["vial_dw"][tubes][:, None]), cest_r1=f32(np.ones((n, 1))),
+                   cest_k=f32(np.where(pool, 900.0, 0.0)[:, None]), cest_f=f32(np.where(pool, 0.008, 0.0)[:, None]),
+                   cest_dw=f32(np.where(pool, data["vial_dw"][tubes], 0.0)[:, None]), cest_r1=f32(np.ones((n, 1))),
                    cest_r2=f32(np.full((n, 1), 1.0 / PHANTOM_POOL_T2)), mt_k=f32(0 * one), mt_f=f32(0 * one),
                    mt_dw=f32(0 * one), mt_r1=f32(0 * one), mt_r2=f32(one), mt_lineshape=np.zeros(n, np.int32),
                    b0=f32(float(data["b0_T"]) * one), gamma=f32(267.5153 * one),
@@ -498,6 +501,69 @@ def bench_phantom(args, kind):
 
 
 REF_JOINT_FIT_S = 7200.0    # examples/README.md:65 of the reference: about 2 h on one RTX 3070
+REF_PER_RUNG_FIT_S = 12600.0  # examples/README.md:66: about 3.5 h on one RTX 3070
+
+
+def bench_phantom_joint(args, per_rung):
+    """The reference's tutorial 10 (examples/10_shared_water.py) and, with per_rung, 11 (11_per_rung.py)
+    on the shipped phantom measurement: k, f, pool T2 of the six pool tubes (fit, 60 iterations), then
+    fit_joint over all seven tubes with the water R2 factor, M_start and a 15-node B0 spread shared (12
+    steps), then (11) the same with one B1 factor per rung shared by the tubes (12 steps)."""
+    import torch
+    from dataclasses import replace
+    from bmc_jax.fitting import FitSpec, FreeParam, fit, fit_joint
+    from bmc_jax.simulation import B0Spread
+    ladder6, spec3, Z6, _, names6, npb = phantom_experiment("quesp", "pool")
+    ladder, _, Z, noise, names, _ = phantom_experiment("quesp", "all")
+    pool = [names.index(nm) for nm in names6]
+    K, F, T2 = spec3.slots
+    R2F = FreeParam("r2w", "water_r2", None, scale=1.0, lo=0.5, hi=2.0, x0=1.0, scope="shared", mode="factor")
+    MS = FreeParam("m_start", "m_start", None, scale=1.0, lo=0.2, hi=1.0, x0=0.557, scope="shared")
+    SIG = FreeParam("sigma", "b0_spread", None, scale=10.0, lo=0.0, hi=8.0, x0=5.0, scope="shared")
+    B1R = FreeParam("b1", "rel_b1", None, scale=1.0, lo=0.7, hi=1.3, x0=1.0, scope="block", mode="factor")
+    spread = B0Spread(nodes=15)
+    Kp, Fp, T2p = (replace(sl, samples=tuple(pool)) for sl in (K, F, T2))
+    spec10 = FitSpec([Kp, Fp, T2p, R2F, MS, SIG])
+    fit(ladder6, spec3, Z6, n_iter=2, n_per_block=npb)            # warm: plans, graph, allocator
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = fit(ladder6, spec3, Z6, n_iter=60, n_per_block=npb)
+    x0 = np.tile([sl.x0 for sl in spec10.slots], (len(names), 1))
+    x0[pool, :3] = plain.x
+    water = fit_joint(ladder, spec10, Z, n_iter=12, n_per_block=npb, x0_phys=x0, spread=spread, max_chunk=3)
+    torch.cuda.synchronize()
+    t10 = time.perf_counter() - t0
+    out = {"tutorial10": {"seconds": t10, "shared": {"water_r2_factor": float(water["r2w"][0]),
+                                                     "m_start": float(water["m_start"][0]),
+                                                     "b0_spread_hz": float(water["sigma"][0])},
+                          "cost_first_last": [float(water.cost[0]), float(water.cost[-1])],
+                          "rms_over_noise": (water.rms / noise).tolist(),
+                          "pair_checks": phantom_pairs(np.asarray(water["f"]), names)}}
+    ref, secs, label = REF_JOINT_FIT_S, t10, "10 (shared water)"
+    if per_rung:
+        t1 = time.perf_counter()
+        x1 = np.column_stack([water.x, np.ones(len(names))])
+        rung = fit_joint(ladder, FitSpec([Kp, Fp, T2p, R2F, MS, SIG, B1R]), Z, n_iter=12, n_per_block=npb,
+                         x0_phys=x1, spread=spread, max_chunk=3, block_jacobian="split")
+        torch.cuda.synchronize()
+        t11 = time.perf_counter() - t1
+        b1 = rung.blocks["b1"]
+        out["tutorial11"] = {"seconds_after_10": t11, "b1_per_rung": b1["x"].tolist(), "b1_sd": b1["sd"].tolist(),
+                             "shared": {"water_r2_factor": float(rung["r2w"][0]), "m_start": float(rung["m_start"][0]),
+                                        "b0_spread_hz": float(rung["sigma"][0])},
+                             "pair_checks": phantom_pairs(np.asarray(rung["f"]), names)}
+        ref, secs, label = REF_PER_RUNG_FIT_S, t10 + t11, "11 (shared water, then B1 per rung)"
+    line = {"metric": f"measured-phantom joint fit wall time, reference tutorial {label}: 7 tubes, 15-node B0 "
+                      "spread, drop-in fit() and fit_joint()",
+            "value": secs, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * secs,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": secs / ref,
+            "vs_baseline_note": f"reference's published ~{ref / 3600:.1f} h on one RTX 3070 (examples/README.md:60-66)",
+            "dtype": "f64 simulate (f32 I/O), f64 LM steps",
+            "data": "the reference's shipped 2026-09-27 phantom scan and converted scanner sequences",
+            "config": {"workload": "phantom joint", "tubes": len(names), "spread_nodes": 15, "joint_steps": 12}}
+    line.update(out)
+    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_joint(args):
@@ -738,7 +804,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer (the reference point for scaling HBM counters)")
     ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven", "scanner", "joint", "tutorial08",
-                                           "tutorial09"), default="pulsed",
+                                           "tutorial09", "tutorial10", "tutorial11"), default="pulsed",
                     help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
     ap.add_argument("--nn-train", type=int, default=2_000_000)
     ap.add_argument("--nn-test", type=int, default=262_144)
@@ -770,6 +836,10 @@ def main():
     if args.workload == "scanner":
         _native.torch_cuda()
         bench_scanner(args)
+        return
+    if args.workload in ("tutorial10", "tutorial11"):
+        _native.torch_cuda()
+        bench_phantom_joint(args, args.workload == "tutorial11")
         return
     if args.workload in ("tutorial08", "tutorial09"):
         _native.torch_cuda()

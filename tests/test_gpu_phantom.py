@@ -64,3 +64,32 @@ def test_tutorial_09_fits_the_phantom():
     assert np.all(res.rms / noise < 15.0), res.rms / noise
     pairs = {p["pair"]: p["f_ratio"] for p in bench.phantom_pairs(np.asarray(res["f"]), names)}
     assert pairs["GAG 100 / 50 mg/mL"] == pytest.approx(2.0, rel=0.1), pairs
+
+
+def test_tutorial_10_shares_the_water_over_seven_tubes():
+    """fit_joint over the seven tubes (PBS included) with the water R2 factor, M_start and a 15-node B0
+    spread shared: the joint cost falls below the start's, the shared values stay inside their bounds,
+    and the GAG pair keeps its ratio of 2."""
+    import bench
+    from dataclasses import replace
+    from bmc_jax.fitting import FitSpec, FreeParam, fit, fit_joint
+    from bmc_jax.simulation import B0Spread
+    ladder6, spec3, Z6, _, names6, npb = bench.phantom_experiment("quesp", "pool")
+    ladder, _, Z, noise, names, _ = bench.phantom_experiment("quesp", "all")
+    pool = [names.index(nm) for nm in names6]
+    plain = fit(ladder6, spec3, Z6, n_iter=60, n_per_block=npb)
+    Kp, Fp, T2p = (replace(sl, samples=tuple(pool)) for sl in spec3.slots)
+    spec = FitSpec([Kp, Fp, T2p,
+                    FreeParam("r2w", "water_r2", None, 1.0, 0.5, 2.0, 1.0, scope="shared", mode="factor"),
+                    FreeParam("m_start", "m_start", None, 1.0, 0.2, 1.0, 0.557, scope="shared"),
+                    FreeParam("sigma", "b0_spread", None, 10.0, 0.0, 8.0, 5.0, scope="shared")])
+    x0 = np.tile([sl.x0 for sl in spec.slots], (len(names), 1))
+    x0[pool, :3] = plain.x
+    res = fit_joint(ladder, spec, Z, n_iter=12, n_per_block=npb, x0_phys=x0, spread=B0Spread(nodes=15))
+    assert res.cost[-1] < res.cost[0] and res.nan_steps == 0
+    for name in ("r2w", "m_start", "sigma"):
+        v = res[name]
+        assert np.all(v == v[0]) and np.isfinite(v[0])
+    pairs = {p["pair"]: p["f_ratio"] for p in bench.phantom_pairs(np.asarray(res["f"]), names)}
+    assert pairs["GAG 100 / 50 mg/mL"] == pytest.approx(2.0, rel=0.1), pairs
+    assert not res.free[names.index("PBS"), 0]          # PBS has no pool: its k is held
