@@ -20,6 +20,7 @@ SIM_FORWARD_JACOBIAN = 1
 ABI_VERSION = 2
 MAX_CEST = 4
 MAX_TANGENTS = 16
+EXPM_MAX_N = 24
 
 # packed record layout (include/bmc_hip.h)
 SCALAR_SLOTS = {"water_r1": 0, "water_r2": 1, "water_f": 2, "mt_k": 3, "mt_f": 4, "mt_dw": 5, "mt_r1": 6,
@@ -84,7 +85,8 @@ def lib():
         L.bmc_adjoint_workspace_size.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, i64p]
         L.bmc_simulate_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int, ip32,
                                            dp, ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int64, vp]
-        for name in ("bmc_plan_create", "bmc_plan_create_ex", "bmc_plan_destroy", "bmc_plan_stats", "bmc_plan_program",
+        L.bmc_expm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int, vp]
+        for name in ("bmc_expm", "bmc_plan_create", "bmc_plan_create_ex", "bmc_plan_destroy", "bmc_plan_stats", "bmc_plan_program",
                      "bmc_plan_compile",
                      "bmc_simulate", "bmc_simulate_ex", "bmc_simulate_trajectory", "bmc_lm_fit", "bmc_lm_fit_noalloc", "bmc_adjoint_workspace_size",
                      "bmc_simulate_adjoint"):
@@ -211,6 +213,25 @@ def device_array(x, dtype):
     torch = torch_cuda()
     t = torch.as_tensor(np.ascontiguousarray(x))
     return t.to(device="cuda", dtype=dtype).contiguous()
+
+
+def expm(a):
+    """bmc_expm: exp of every trailing n x n matrix of `a` (numpy, float32 or float64, any leading axes),
+    computed on the GPU in a's precision; returned as numpy of a's dtype and shape."""
+    torch = torch_cuda()
+    a = np.asarray(a)
+    if a.dtype not in (np.float32, np.float64):
+        raise ValueError(f"expm takes float32 or float64 matrices, not {a.dtype}")
+    if a.ndim < 2 or a.shape[-1] != a.shape[-2]:
+        raise ValueError(f"expm takes square matrices in the last two axes, not shape {a.shape}")
+    n = a.shape[-1]
+    batch = int(np.prod(a.shape[:-2], dtype=np.int64))
+    f64 = a.dtype == np.float64
+    t = device_array(a.reshape(batch, n, n), torch.float64 if f64 else torch.float32)
+    out = torch.empty_like(t)
+    check(lib().bmc_expm(ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(out.data_ptr()), n, batch,
+                         BMC_F64 if f64 else BMC_F32, stream_ptr()), "bmc_expm")
+    return out.cpu().numpy().reshape(a.shape)
 
 
 def simulate(plan: Plan, params, n_cest: int, dtype_code: int, m0, dm0=None, slots=(), scales=(), jc=0,

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "bmc_jax", "_lib")
 LIB = os.path.join(OUT_DIR, "libbmc_hip.so")
 SOURCES = ["bmc_abi.hip", "bmc_sim.hip", "bmc_cwj.hip", "bmc_adj.hip", "bmc_uneven.hip", "bmc_lm.hip",
-           "bmc_mlp.hip"]
+           "bmc_mlp.hip", "bmc_expm.hip"]
 ARCH = os.environ.get("BMC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -294,6 +294,16 @@ int bmc_mlp_fit_input_norm(bmc_mlp* mlp, const float* x_dev, int64_t n, void* st
 int bmc_mlp_set_input_norm(bmc_mlp* mlp, const float* shift_host, const float* scale_host);
 int bmc_mlp_get_input_norm(const bmc_mlp* mlp, float* shift_host, float* scale_host);
 
+/* Dense matrix exponential exp(A) of `batch` row-major n x n matrices (device, float32 or float64 by
+ * dtype), for callers that hold the matrix themselves: replaces expm_library / expm_plain
+ * (src/bmc_jax/simulation/expm.py:26-83), on which propagate (:86-119) steps. Pade approximation with
+ * scaling and squaring in the reference's rule: s = max(0, floor(log2(||A||_1 / theta))), the Pade order
+ * from ||A||_1 (float64: 3, 5, 7, 9, 13; float32: 3, 5, 7, computed in float32), Gauss-Jordan solve with
+ * partial pivoting, s squarings; more than 16 squarings, or a non-finite norm, gives a NaN matrix.
+ * One workgroup per matrix; n <= BMC_EXPM_MAX_N. */
+#define BMC_EXPM_MAX_N 24
+int bmc_expm(const void* a_dev, void* out_dev, int n, int64_t batch, int dtype, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,3 +39,13 @@ def test_errors_are_reported_without_a_device():
     rc = L.bmc_plan_create_ex(0, 1, None, None, None, None, None, 1, 1, ctypes.byref(h))
     assert rc == -1 and b"bad arguments" in L.bmc_last_error()
     assert L.bmc_plan_destroy(None) == 0
+
+
+def test_expm_argument_errors_without_a_device():
+    """bmc_expm refuses sizes outside 1..BMC_EXPM_MAX_N and unknown dtypes before touching the device."""
+    from bmc_jax import _native
+    L = _native.lib()
+    assert L.bmc_expm(None, None, 25, 1, 1, None) == -1 and b"n must be" in L.bmc_last_error()
+    assert L.bmc_expm(None, None, 0, 1, 1, None) == -1
+    assert L.bmc_expm(None, None, 4, 1, 7, None) == -1 and b"dtype" in L.bmc_last_error()
+    assert L.bmc_expm(None, None, 4, 0, 1, None) == 0

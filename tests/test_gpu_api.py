@@ -148,6 +148,32 @@ def test_exchange_rate_and_fraction_are_recovered(fitted):
     assert (X[:, 0] >= spec.lo_phys[0]).all() and (X[:, 0] <= spec.hi_phys[0]).all()
 
 
+def test_readme_quick_start(bj):
+    """The reference's README quick start (README.md:61-94), run as printed: an amide QUESP ladder simulated
+    and fitted back with fit(). The README prints k = 49.617, f = 0.00090772, rms 2.06e-05 against the truth
+    k = 50, f = 9.01e-4, and says the last digits depend on device and precision; the band holds both."""
+    from bmc_jax.fitting import Experiment, FitSpec, FreeParam, fit
+    from bmc_jax.io import example_config
+    from bmc_jax.sequences import cw_saturation
+    from bmc_jax.simulation import initial_state, to_sim_params
+    params = example_config("amide")
+    jp = to_sim_params(params)
+    m0 = initial_state(jp)
+    ppm = np.linspace(-6, 6, 61)
+    ladder = Experiment.stack([
+        Experiment(jp, cw_saturation(ppm, b1_uT=b, tsat_s=2.0, b0_T=params.system.b0), m0, params.mz_loc)
+        for b in (0.5, 1.0, 2.0, 4.0)])
+    z = np.asarray(ladder.simulate())
+    spec = FitSpec([FreeParam("k", "cest_k", 0, scale=100.0, lo=5.0, hi=5000.0, x0=300.0),
+                    FreeParam("f", "cest_f", 0, scale=0.001, lo=1e-6, hi=0.05, x0=0.0005)])
+    res = fit(ladder, spec, z[None, :], n_per_block=len(ppm))
+    k, f = float(res["k"][0]), float(res["f"][0])
+    assert 49.0 <= k <= 50.5, res.summary()
+    assert 8.9e-4 <= f <= 9.15e-4, res.summary()
+    assert float(res.rms[0]) <= 5e-5, res.summary()
+    assert "sample" in res.summary() and "rms" in res.summary()
+
+
 def test_levenberg_marquardt_without_physics(bj):
     from bmc_jax.fitting import levenberg_marquardt
     r = lambda x: np.array([10.0 * (x[1] - x[0] ** 2), 1.0 - x[0]])
