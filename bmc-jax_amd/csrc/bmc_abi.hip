@@ -483,8 +483,8 @@ int fill_sim_args(const bmc_plan* plan, int V, int nc, int dtype, const void* pa
 #ifndef BMC_TANGENTS_PER_PASS
 #define BMC_TANGENTS_PER_PASS 2
 #endif
-#if BMC_TANGENTS_PER_PASS < 1 || BMC_TANGENTS_PER_PASS > 2
-#error "the simulate kernels are instantiated for 1 or 2 tangents per pass"
+#if BMC_TANGENTS_PER_PASS < 1 || BMC_TANGENTS_PER_PASS > 5
+#error "the simulate kernels are instantiated for 1 to 5 tangents per pass"
 #endif
 int tangents_per_pass(int P) { return P <= 0 ? 0 : (P < BMC_TANGENTS_PER_PASS ? P : BMC_TANGENTS_PER_PASS); }
 

@@ -354,6 +354,133 @@ __device__ __forceinline__ void apply_dX(int kind, int ix, const TanCo& t, const
   }
 }
 
+// The generator derivative of one tangent in one of four shapes that together cover every tangent kind,
+// with the shape's coefficients q (already times dt 2^-s):
+//   WATER  water / MT relaxation, exchange and saturation terms (W_R1, W_R2, MT_*):
+//            y[xw] += q0 v[xw], y[yw] += q0 v[yw], y[zw] += q1 v[zw] + q2 v[zmt] + q3 vaug,
+//            y[zmt] += q4 v[zw] + q5 v[zmt] + q6 vaug
+//   POOL   one CEST pool ix (C_K, C_F, C_DW, C_R1, C_R2), c = its (x, y, z):
+//            y[w] += q0 v[w] + q1 v[c] (w = xw, yw, zw and c = xc, yc, zc in turn),
+//            y[xc] += q2 v[xw] + q3 v[xc] + q4 v[yc], y[yc] += q2 v[yw] + q3 v[yc] - q4 v[xc],
+//            y[zc] += q2 v[zw] + q5 v[zc] + q6 vaug
+//   SHIFT  a change of every offset (B0INH, B0, GAMMA): y[x] += g v[y], y[y] -= g v[x] for water
+//            (g = q0) and pool i (g = q[1 + i]), y[zmt] += q6 v[zmt]
+//   B1     a change of the RF vector (RELB1): y[x] -= q1 v[z], y[y] += q0 v[z], y[z] += q1 v[x] - q0 v[y]
+//            for water and every pool, y[zmt] += q2 v[zmt]
+// The shape is found once per event (dx_shape); the Taylor loop then applies it with a four-way branch
+// instead of dispatching on the tangent kind at every term.
+enum DxShape { DXS_NONE = 0, DXS_WATER, DXS_POOL, DXS_SHIFT, DXS_B1 };
+struct DxCo {
+  int shape, ix;
+  double q[7];
+};
+
+template <int NC>
+__device__ __forceinline__ DxCo dx_shape(int kind, int ix, const TanCo& t, const double* dwp) {
+  DxCo d;
+  d.shape = DXS_NONE;
+  d.ix = ix;
+#pragma unroll
+  for (int i = 0; i < 7; i++) d.q[i] = 0.0;
+  switch (kind) {
+    case TK_W_R1:
+      d.shape = DXS_WATER, d.q[1] = -t.c0, d.q[3] = t.c0;
+      break;
+    case TK_W_R2:
+      d.shape = DXS_WATER, d.q[0] = -t.c0;
+      break;
+    case TK_MT_K:
+      d.shape = DXS_WATER, d.q[1] = -t.c0, d.q[2] = t.c1, d.q[4] = t.c0, d.q[5] = -t.c1;
+      break;
+    case TK_MT_F:
+      d.shape = DXS_WATER, d.q[1] = -t.c0, d.q[4] = t.c0, d.q[6] = t.c1;
+      break;
+    case TK_MT_R1:
+      d.shape = DXS_WATER, d.q[5] = -t.c0, d.q[6] = t.c1;
+      break;
+    case TK_MT_DW:
+    case TK_MT_R2:
+      d.shape = DXS_WATER, d.q[5] = t.c2;
+      break;
+    case TK_B0INH:
+    case TK_B0:
+    case TK_GAMMA:
+      // water: d off_w = c0; pool i: d o_i = dw_i c1 - c0 (c1 = 0 for B0INH)
+      d.shape = DXS_SHIFT, d.q[0] = t.c0, d.q[6] = t.c2;
+#pragma unroll
+      for (int i = 0; i < NC; i++) d.q[1 + i] = (kind == TK_B0INH) ? t.c0 : fma(-dwp[i], t.c1, t.c0);
+      break;
+    case TK_RELB1:
+      d.shape = DXS_B1, d.q[0] = t.c0, d.q[1] = t.c1, d.q[2] = t.c2;
+      break;
+    case TK_C_K:  // d k = 1 (c1), d kw = f (c0)
+      d.shape = DXS_POOL, d.q[0] = -t.c0, d.q[1] = t.c1, d.q[2] = t.c0, d.q[3] = -t.c1, d.q[5] = -t.c1;
+      break;
+    case TK_C_F:  // d kw = k (c0), d C_zc = r1c (c1)
+      d.shape = DXS_POOL, d.q[0] = -t.c0, d.q[2] = t.c0, d.q[6] = t.c1;
+      break;
+    case TK_C_DW:  // d o_i = c0
+      d.shape = DXS_POOL, d.q[4] = -t.c0;
+      break;
+    case TK_C_R1:
+      d.shape = DXS_POOL, d.q[5] = -t.c0, d.q[6] = t.c1;
+      break;
+    case TK_C_R2:
+      d.shape = DXS_POOL, d.q[3] = -t.c0;
+      break;
+    default:
+      break;
+  }
+  return d;
+}
+
+// y += dX v for a tangent in shape form (the same values as apply_dX with the kind it came from).
+template <int NC>
+__device__ __forceinline__ void apply_dx_shape(const DxCo& d, const double* v, double vaug, double* y) {
+  constexpr int NT = 1 + NC;
+  constexpr int xw = 0, yw = NT, zw = 2 * NT, zmt = 3 * NT;
+  const double* q = d.q;
+  if (d.shape == DXS_WATER) {
+    y[xw] = fma(q[0], v[xw], y[xw]);
+    y[yw] = fma(q[0], v[yw], y[yw]);
+    y[zw] = fma(q[1], v[zw], fma(q[2], v[zmt], fma(q[3], vaug, y[zw])));
+    y[zmt] = fma(q[4], v[zw], fma(q[5], v[zmt], fma(q[6], vaug, y[zmt])));
+  } else if (d.shape == DXS_POOL) {
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+      if (i != d.ix) continue;
+      const int xc = 1 + i, yc = NT + 1 + i, zc = 2 * NT + 1 + i;
+      y[xw] = fma(q[0], v[xw], fma(q[1], v[xc], y[xw]));
+      y[yw] = fma(q[0], v[yw], fma(q[1], v[yc], y[yw]));
+      y[zw] = fma(q[0], v[zw], fma(q[1], v[zc], y[zw]));
+      const double vx = v[xc], vy = v[yc];
+      y[xc] = fma(q[2], v[xw], fma(q[3], vx, fma(q[4], vy, y[xc])));
+      y[yc] = fma(q[2], v[yw], fma(q[3], vy, fma(-q[4], vx, y[yc])));
+      y[zc] = fma(q[2], v[zw], fma(q[5], v[zc], fma(q[6], vaug, y[zc])));
+    }
+  } else if (d.shape == DXS_SHIFT) {
+    y[xw] = fma(q[0], v[yw], y[xw]);
+    y[yw] = fma(-q[0], v[xw], y[yw]);
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+      const int xc = 1 + i, yc = NT + 1 + i;
+      y[xc] = fma(q[1 + i], v[yc], y[xc]);
+      y[yc] = fma(-q[1 + i], v[xc], y[yc]);
+    }
+    y[zmt] = fma(q[6], v[zmt], y[zmt]);
+  } else if (d.shape == DXS_B1) {
+#pragma unroll
+    for (int i = 0; i < NT; i++) {
+      const int xi = i, yi = NT + i, zi = 2 * NT + i;
+      const double vx = v[xi], vy = v[yi], vz = v[zi];
+      y[xi] = fma(-q[1], vz, y[xi]);
+      y[yi] = fma(q[0], vz, y[yi]);
+      y[zi] = fma(q[1], vx, fma(-q[0], vy, y[zi]));
+    }
+    y[zmt] = fma(q[2], v[zmt], y[zmt]);
+  }
+}
+
 // Per-team cache of the MT lineshape, keyed on the event's RF frequency and amplitude presence.
 struct MtCache {
   double fr;
@@ -541,6 +668,9 @@ __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const 
                                        double (*dacc)[3 * (1 + NC) + 1]) {
   constexpr int N = 3 * (1 + NC) + 1;
   double v[N], dv[PT > 0 ? PT : 1][N];
+  DxCo dx[PT > 0 ? PT : 1];
+#pragma unroll
+  for (int p = 0; p < PT; p++) dx[p] = dx_shape<NC>(kind[p], kidx[p], tc[p], dwp);
 #pragma unroll
   for (int i = 0; i < N; i++) v[i] = v0[i];
 #pragma unroll
@@ -553,20 +683,22 @@ __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const 
   for (int p = 0; p < PT; p++)
 #pragma unroll
     for (int i = 0; i < N; i++) dacc[p][i] = dv[p][i];
+  double ik = kInv[m];
   for (int k = m; k >= 1; k--) {
-    const double ik = kInv[k];
+    const double ik_next = kInv[k - 1];  // the next term's 1 / k is in flight during this one
     double y[N];
     apply_X<NC>(G, acc, vaug0, y);
 #pragma unroll
     for (int p = 0; p < PT; p++) {
       double dy[N];
       apply_X<NC>(G, dacc[p], 0.0, dy);
-      apply_dX<NC>(kind[p], kidx[p], tc[p], dwp, acc, vaug0, dy);  // dX at the current iterate
+      apply_dx_shape<NC>(dx[p], acc, vaug0, dy);  // dX at the current iterate
 #pragma unroll
       for (int i = 0; i < N; i++) dacc[p][i] = fma(dy[i], ik, dv[p][i]);
     }
 #pragma unroll
     for (int i = 0; i < N; i++) acc[i] = fma(y[i], ik, v[i]);
+    ik = ik_next;
   }
 }
 

@@ -36,6 +36,10 @@ namespace bmc {
 #ifndef BMC_SIM_WAVES_PER_SIMD
 #define BMC_SIM_WAVES_PER_SIMD 2
 #endif
+// Most tangents one pass carries (the launch switch below instantiates 0..this many).
+#ifndef BMC_TANGENTS_PER_PASS
+#define BMC_TANGENTS_PER_PASS 2
+#endif
 
 // Doubles of one team's keep slot: the propagator and its PT tangents, N rows by N + 1 columns.
 template <int NC, int PT>
@@ -238,8 +242,20 @@ static int launch_nc(const SimArgs& a, hipStream_t stream) {
       return launch_nc_pt<NC, 1>(a, stream);
     case 2:
       return launch_nc_pt<NC, 2>(a, stream);
+#if BMC_TANGENTS_PER_PASS > 2
+    case 3:
+      return launch_nc_pt<NC, 3>(a, stream);
+#endif
+#if BMC_TANGENTS_PER_PASS > 3
+    case 4:
+      return launch_nc_pt<NC, 4>(a, stream);
+#endif
+#if BMC_TANGENTS_PER_PASS > 4
+    case 5:
+      return launch_nc_pt<NC, 5>(a, stream);
+#endif
     default:
-      set_error("simulate: tangents per pass must be 0..2");
+      set_error("simulate: tangents per pass must be 0.." + std::to_string(BMC_TANGENTS_PER_PASS));
       return BMC_EINVAL;
   }
 }
