@@ -434,21 +434,24 @@ __device__ __forceinline__ DxCo dx_shape(int kind, int ix, const TanCo& t, const
   return d;
 }
 
-// y += dX v for a tangent in shape form (the same values as apply_dX with the kind it came from).
-template <int NC>
+// y += dX v for a tangent in shape form (the same values as apply_dX with the kind it came from). With
+// SHAPE >= 0 the shape (and, for DXS_POOL, the pool IX) is fixed at compile time and d's are ignored.
+template <int NC, int SHAPE = -1, int IX = -1>
 __device__ __forceinline__ void apply_dx_shape(const DxCo& d, const double* v, double vaug, double* y) {
   constexpr int NT = 1 + NC;
   constexpr int xw = 0, yw = NT, zw = 2 * NT, zmt = 3 * NT;
   const double* q = d.q;
-  if (d.shape == DXS_WATER) {
+  const int shape = SHAPE >= 0 ? SHAPE : d.shape;
+  const int pix = IX >= 0 ? IX : d.ix;
+  if (shape == DXS_WATER) {
     y[xw] = fma(q[0], v[xw], y[xw]);
     y[yw] = fma(q[0], v[yw], y[yw]);
     y[zw] = fma(q[1], v[zw], fma(q[2], v[zmt], fma(q[3], vaug, y[zw])));
     y[zmt] = fma(q[4], v[zw], fma(q[5], v[zmt], fma(q[6], vaug, y[zmt])));
-  } else if (d.shape == DXS_POOL) {
+  } else if (shape == DXS_POOL) {
 #pragma unroll
     for (int i = 0; i < NC; i++) {
-      if (i != d.ix) continue;
+      if (i != pix) continue;
       const int xc = 1 + i, yc = NT + 1 + i, zc = 2 * NT + 1 + i;
       y[xw] = fma(q[0], v[xw], fma(q[1], v[xc], y[xw]));
       y[yw] = fma(q[0], v[yw], fma(q[1], v[yc], y[yw]));
@@ -458,7 +461,7 @@ __device__ __forceinline__ void apply_dx_shape(const DxCo& d, const double* v, d
       y[yc] = fma(q[2], v[yw], fma(q[3], vy, fma(-q[4], vx, y[yc])));
       y[zc] = fma(q[2], v[zw], fma(q[5], v[zc], fma(q[6], vaug, y[zc])));
     }
-  } else if (d.shape == DXS_SHIFT) {
+  } else if (shape == DXS_SHIFT) {
     y[xw] = fma(q[0], v[yw], y[xw]);
     y[yw] = fma(-q[0], v[xw], y[yw]);
 #pragma unroll
@@ -468,7 +471,7 @@ __device__ __forceinline__ void apply_dx_shape(const DxCo& d, const double* v, d
       y[yc] = fma(-q[1 + i], v[xc], y[yc]);
     }
     y[zmt] = fma(q[6], v[zmt], y[zmt]);
-  } else if (d.shape == DXS_B1) {
+  } else if (shape == DXS_B1) {
 #pragma unroll
     for (int i = 0; i < NT; i++) {
       const int xi = i, yi = NT + i, zi = 2 * NT + i;
@@ -661,7 +664,41 @@ __device__ void build_gen(const double* prm, const Ev& e, MtCache& mc, const int
 // where the power sum needs a scale and an add. The tangents follow the same recursion,
 // dw <- dv0 + (X dw + dX w) / k. The homogeneous coordinate of every iterate stays vaug0 (X's last row
 // is zero). acc / dacc may alias v0 / dv0 (in-place use): v0 is copied first.
-template <int NC, int PT>
+// The Horner recursion itself, over terms k = m..1, with the tangents' dX in shape form (SHAPE / IX as in
+// apply_dx_shape).
+template <int NC, int PT, int SHAPE, int IX>
+__device__ __forceinline__ void taylor_terms(const Gen<NC>& G, const DxCo* dx, int m, const double* v, double vaug0,
+                                             const double (*dv)[3 * (1 + NC) + 1], double* acc,
+                                             double (*dacc)[3 * (1 + NC) + 1]) {
+  constexpr int N = 3 * (1 + NC) + 1;
+  double ik = kInv[m];
+  for (int k = m; k >= 1; k--) {
+    const double ik_next = kInv[k - 1];  // the next term's 1 / k is in flight during this one
+    double y[N];
+    apply_X<NC>(G, acc, vaug0, y);
+#pragma unroll
+    for (int p = 0; p < PT; p++) {
+      double dy[N];
+      apply_X<NC>(G, dacc[p], 0.0, dy);
+      apply_dx_shape<NC, SHAPE, IX>(dx[p], acc, vaug0, dy);  // dX at the current iterate
+#pragma unroll
+      for (int i = 0; i < N; i++) dacc[p][i] = fma(dy[i], ik, dv[p][i]);
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) acc[i] = fma(y[i], ik, v[i]);
+    ik = ik_next;
+  }
+}
+
+// Truncated Taylor series acc = T_m(X) v0 with tangents dacc = d(T_m(X) v0) (v0's own tangent dv0),
+// by Horner's rule: w <- v0 + (X w) / k for k = m, ..., 1 from w = v0, one FMA per entry and term
+// where the power sum needs a scale and an add. The tangents follow the same recursion,
+// dw <- dv0 + (X dw + dX w) / k. The homogeneous coordinate of every iterate stays vaug0 (X's last row
+// is zero). acc / dacc may alias v0 / dv0 (in-place use): v0 is copied first.
+// UNSWITCH: when every tangent has the same dX shape (and pool), run a copy of the recursion with that
+// shape fixed at compile time, so no term branches on it (for the hottest call site only: each copy is
+// more code).
+template <int NC, int PT, bool UNSWITCH = false>
 __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
                                        const double* dwp, int m, const double* v0, double vaug0,
                                        const double (*dv0)[3 * (1 + NC) + 1], double* acc,
@@ -683,23 +720,28 @@ __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const 
   for (int p = 0; p < PT; p++)
 #pragma unroll
     for (int i = 0; i < N; i++) dacc[p][i] = dv[p][i];
-  double ik = kInv[m];
-  for (int k = m; k >= 1; k--) {
-    const double ik_next = kInv[k - 1];  // the next term's 1 / k is in flight during this one
-    double y[N];
-    apply_X<NC>(G, acc, vaug0, y);
+  if constexpr (UNSWITCH && PT > 0) {
+    bool same = true;
 #pragma unroll
-    for (int p = 0; p < PT; p++) {
-      double dy[N];
-      apply_X<NC>(G, dacc[p], 0.0, dy);
-      apply_dx_shape<NC>(dx[p], acc, vaug0, dy);  // dX at the current iterate
-#pragma unroll
-      for (int i = 0; i < N; i++) dacc[p][i] = fma(dy[i], ik, dv[p][i]);
+    for (int p = 1; p < PT; p++) same = same && dx[p].shape == dx[0].shape && dx[p].ix == dx[0].ix;
+    if (same) {
+      const int sh = dx[0].shape;
+      if (sh == DXS_WATER) return taylor_terms<NC, PT, DXS_WATER, -1>(G, dx, m, v, vaug0, dv, acc, dacc);
+      if (sh == DXS_SHIFT) return taylor_terms<NC, PT, DXS_SHIFT, -1>(G, dx, m, v, vaug0, dv, acc, dacc);
+      if (sh == DXS_B1) return taylor_terms<NC, PT, DXS_B1, -1>(G, dx, m, v, vaug0, dv, acc, dacc);
+      if (sh == DXS_POOL) {
+        if constexpr (NC > 0)
+          if (dx[0].ix == 0) return taylor_terms<NC, PT, DXS_POOL, 0>(G, dx, m, v, vaug0, dv, acc, dacc);
+        if constexpr (NC > 1)
+          if (dx[0].ix == 1) return taylor_terms<NC, PT, DXS_POOL, 1>(G, dx, m, v, vaug0, dv, acc, dacc);
+        if constexpr (NC > 2)
+          if (dx[0].ix == 2) return taylor_terms<NC, PT, DXS_POOL, 2>(G, dx, m, v, vaug0, dv, acc, dacc);
+        if constexpr (NC > 3)
+          if (dx[0].ix == 3) return taylor_terms<NC, PT, DXS_POOL, 3>(G, dx, m, v, vaug0, dv, acc, dacc);
+      }
     }
-#pragma unroll
-    for (int i = 0; i < N; i++) acc[i] = fma(y[i], ik, v[i]);
-    ik = ik_next;
   }
+  taylor_terms<NC, PT, -1, -1>(G, dx, m, v, vaug0, dv, acc, dacc);
 }
 
 // out = A B for affine propagators in column layout (lane j holds column j, rows 0..N-1):
@@ -965,7 +1007,7 @@ __device__ __forceinline__ void run_row(const double* prm, const double* st0, do
           if (s == 0) {
             // exp(X) Pi = T_m(X) Pi: each lane runs the series on its own column of the chained
             // propagator (homogeneous coordinate Pi[N][j] = [j == N]); no dense product is needed
-            taylor<NC, PT>(G, tc, kind, kidx, dwp, m, Pi, (lane == N) ? 1.0 : 0.0, dPi, Pi, dPi);  // in place
+            taylor<NC, PT, true>(G, tc, kind, kidx, dwp, m, Pi, (lane == N) ? 1.0 : 0.0, dPi, Pi, dPi);  // in place
             if (isn) {
 #pragma unroll
               for (int i = 0; i < N; i++) {
