@@ -290,7 +290,7 @@ def bench_uneven(args):
             fl += float(_flops.reverse_flops(pack(pw, 1), nc, ev_i, spec_slots(p.spec, nc), live[r0:r0 + nr])[0])
         else:
             fl += float(_flops.executed_flops(pack(pw, 1), nc, ev_i, ops, rp[r0:r0 + nr + 1], spec_slots(p.spec, nc),
-                                              pt=best[0])[0])
+                                              pt=best[0], mirror=False)[0])
     flops_sweep = fl / len(sample) * b.n
     achieved = flops_sweep / (sweep_ms * 1e-3) / 1e12
     # one bmc_lm_fit per problem (what a per-shape loop of `fit` calls launches), on a sample
@@ -810,6 +810,8 @@ def main():
     ap.add_argument("--nn-test", type=int, default=262_144)
     ap.add_argument("--nn-epochs", type=int, default=30)
     ap.add_argument("--no-nn", action="store_true", help="leave the learned-proposal sub-record out of the line")
+    ap.add_argument("--plan-level", type=int, default=3, choices=(2, 3),
+                    help="pulsed workload's plan: 3 factorises the symmetric pulses (default), 2 chains every event")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (voxels are independent: "
@@ -868,7 +870,7 @@ def main():
     prm, m0, ev, nc = workload(args.voxels)
     slots = [_native.slot_index(f, i, nc) for f, i in FREE]
     scales = [100.0, 0.001, 1.0, 10.0, 0.1]
-    plan = _engine.plan_for(SimEvent(*ev))
+    plan = _engine.plan_for(SimEvent(*ev), compress=args.plan_level)
     prm_d = _native.device_array(prm, torch.float32)
     m0_d = _native.device_array(m0, torch.float32)
     jc = 2 * (1 + nc)

@@ -28,8 +28,9 @@ def _events_key(events, reset_after_adc, compress):
 
 
 def plan_for(events, reset_after_adc=True, compress=True) -> _native.Plan:
-    """A device plan for an event table, cached on its exact contents. compress: True / 2 (runs and
-    phase-shifted pulses reused), 1 (runs only), False / 0 (every event its own step)."""
+    """A device plan for an event table, cached on its exact contents. compress: True / 3 (runs,
+    phase-shifted pulses reused and palindromic runs factorised), 2 (no factorisation), 1 (runs only),
+    False / 0 (every event its own step)."""
     key = _events_key(events, reset_after_adc, compress)
     plan = _PLANS.get(key)
     if plan is not None:

@@ -8,6 +8,11 @@ count s (the same 1-norm and theta_m table) and counts the useful float64 operat
   dense product (column)    2 N^2 (1 + 2 PT) per column, N + 1 useful columns (squarings, and chaining
                             an event that needed squarings; other chained events need none)
   apply to the state        (N + 1)(N + 3 PT N) + N (1 + PT) N adds of the team reduction
+  palindrome factorisation  a chained run opening with 2h mirrored events (plan level 3; teams with a
+                            spare lane, plans without kept pulses) chains h events on N + 2 columns
+                            (the spare lane's reversed chain included) and recombines them with
+                            2 N^2 (1 + 2 PT) flops per column, when the voxel qualifies (mirror_ok in
+                            csrc/bmc_device.h) and no event of the half needs squarings
 Only the N + 1 columns of a propagator count, not the idle lanes of a 16-lane team.
 `reference_flops` is a model of the reference's per-event work (every event its own
 Pade exponential with its tangents, reference float32 degree/squaring rule).
@@ -101,11 +106,35 @@ def degree_and_squarings(l1, mmax=M_MAX):
     return m.astype(np.int64), s
 
 
-def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1):
+def team_of(nc):
+    """Lanes per team of the simulate kernel (one per propagator column, csrc/bmc_sim.hip)."""
+    na = 3 * (1 + nc) + 2
+    return 8 if na <= 8 else 12 if na <= 12 else 16 if na <= 16 else 32
+
+
+def mirror_ok(prm, nc, slots):
+    """Per voxel of prm [V, S]: whether the parameters and the pass's tangent slots admit the palindrome
+    factorisation (csrc/bmc_device.h mirror_ok): every pool and MT has a positive fraction or is
+    decoupled (fraction and rate 0) with no tangent along its rate."""
+    fmt, kmt = prm[:, 4], prm[:, 3]
+    ok = (fmt > 0) | ((fmt == 0) & (kmt == 0))
+    if 3 in slots:                                     # mt_k
+        ok &= fmt > 0
+    for i in range(nc):
+        f, k = prm[:, 13 + nc + i], prm[:, 13 + i]
+        ok &= (f > 0) | ((f == 0) & (k == 0))
+        if 13 + i in slots:                            # cest_k of pool i
+            ok &= f > 0
+    return ok
+
+
+def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1, mirror=True):
     """Useful float64 flops of one bmc_simulate call, per voxel of prm [V, S] (summed over offsets and passes).
 
     events: dict/SimEvent of (O, T) arrays; ops/row_ptr: the plan's program (kinds 0 SEQ, 1 BLOCK,
-    2 KEEP counted as a BLOCK, 3 REUSE: the kept propagator applied once, rotated); slots: tangent slots.
+    2 KEEP counted as a BLOCK, 3 REUSE: the kept propagator applied once, rotated; column 4 the
+    palindromic half); slots: tangent slots. mirror=False for kernels that chain every event (the
+    uneven batch).
     """
     ev = {k: np.asarray(getattr(events, k) if not isinstance(events, dict) else events[k])
           for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
@@ -117,9 +146,13 @@ def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1):
     V = prm.shape[0]
     total = np.zeros(V)
     O = ev["dt"].shape[0]
+    ops = np.asarray(ops)
+    can_mirror = (mirror and ops.shape[1] > 4 and team_of(nc) > NA
+                  and not np.any(ops[:, 0] == 2))     # kernels with kept pulses have no stage
     for o in range(O):
         for oi in range(row_ptr[o], row_ptr[o + 1]):
-            kind, e0, ln, cnt = (int(v) for v in ops[oi])
+            kind, e0, ln, cnt = (int(v) for v in ops[oi][:4])
+            half = int(ops[oi][4]) if ops.shape[1] > 4 else 0
             if kind == 3:                 # REUSE: the kept propagator applied once, rotated (2 x 6 flops per pool)
                 for ps in passes:
                     T = len(ps)
@@ -149,7 +182,16 @@ def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1):
                     # chained: an event without squarings runs its series on the chained columns
                     # directly; one with squarings forms its propagator and multiplies it in
                     f = np.where(s == 0, NA * m * term, NA * m * term + s * mul + mul)
-                    total += f.sum(axis=1) + cnt * app
+                    plain = f.sum(axis=1)
+                    if can_mirror and half > 0:
+                        # events [e0, e0 + 2h) with dt > 1e-9 are the first live ones of sel
+                        nh = int(np.sum(live[:half]))
+                        n2 = int(np.sum(live[:2 * half]))
+                        ok = mirror_ok(prm, nc, ps) & np.all(s[:, :nh] == 0, axis=1)
+                        mir = (NA + 1) * (m[:, :nh] * term).sum(axis=1) + NA * 2 * N * N * (1 + 2 * T) \
+                            + f[:, n2:].sum(axis=1)
+                        plain = np.where(ok, mir, plain)
+                    total += plain + cnt * app
     return total
 
 

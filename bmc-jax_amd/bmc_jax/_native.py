@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libbmc_hip.so")
 BMC_F32, BMC_F64 = 0, 1
 LM_EARLY_EXIT, LM_GRAPH, LM_START_CHECK, LM_FORWARD_JACOBIAN = 1, 2, 4, 8
 SIM_FORWARD_JACOBIAN = 1
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_CEST = 4
 MAX_TANGENTS = 16
 EXPM_MAX_N = 24
@@ -122,21 +122,22 @@ def _dp(a):
 
 
 def _level(compress):
-    """Plan compression level from a bool (True = 2) or an int 0..2."""
+    """Plan compression level from a bool (True = 3) or an int 0..3."""
     if compress is True:
-        return 2
+        return 3
     if compress is False or compress is None:
         return 0
     lv = int(compress)
-    if lv not in (0, 1, 2):
-        raise ValueError(f"compression level must be 0, 1 or 2, got {compress}")
+    if lv not in (0, 1, 2, 3):
+        raise ValueError(f"compression level must be 0, 1, 2 or 3, got {compress}")
     return lv
 
 
-def compile_program(type_, dt, rf_amp, rf_phase, rf_freq, compress=2):
+def compile_program(type_, dt, rf_amp, rf_phase, rf_freq, compress=3):
     """The row programs of an event table, compiled on the host only (bmc_plan_compile; needs no GPU):
-    (ops [n_ops, 4] int32 = (kind, first event, length, count), rot [n_ops, 2] = (cos d, sin d),
-    row_ptr [O + 1] int32). Kinds: 0 SEQ, 1 BLOCK, 2 KEEP, 3 REUSE."""
+    (ops [n_ops, 5] int32 = (kind, first event, length, count, half), rot [n_ops, 2] = (cos d, sin d),
+    row_ptr [O + 1] int32). Kinds: 0 SEQ, 1 BLOCK, 2 KEEP, 3 REUSE. half h > 0: the op's first 2h events
+    form a palindrome at one RF phase (its cos, sin in rot), chained from their first half."""
     L = lib()
     t = np.ascontiguousarray(np.asarray(type_), dtype=np.int32)
     if t.ndim != 2:
@@ -147,7 +148,7 @@ def compile_program(type_, dt, rf_amp, rf_phase, rf_freq, compress=2):
     n = ctypes.c_int64()
     args = [O, T, ip(t)] + [_dp(a) for a in f] + [_level(compress)]
     check(L.bmc_plan_compile(*args, 0, None, None, None, ctypes.byref(n)), "bmc_plan_compile")
-    ops = np.zeros((max(n.value, 1), 4), np.int32)
+    ops = np.zeros((max(n.value, 1), 5), np.int32)
     rot = np.zeros((max(n.value, 1), 2), np.float64)
     rp = np.zeros(O + 1, np.int32)
     check(L.bmc_plan_compile(*args, max(n.value, 1), ip(ops), _dp(rot), ip(rp), ctypes.byref(n)), "bmc_plan_compile")
@@ -157,9 +158,10 @@ def compile_program(type_, dt, rf_amp, rf_phase, rf_freq, compress=2):
 class Plan:
     """A sequence compiled for the device (bmc_plan_create): events [O, T] and the row programs."""
 
-    def __init__(self, type_, dt, rf_amp, rf_phase, rf_freq, reset_after_adc=True, compress=2):
-        """compress: 0 every event its own step; 1 runs of identical events chained; 2 (True) also pulses
-        that recur with a shifted RF phase kept and reused (bmc_plan_create_ex)."""
+    def __init__(self, type_, dt, rf_amp, rf_phase, rf_freq, reset_after_adc=True, compress=3):
+        """compress: 0 every event its own step; 1 runs of identical events chained; 2 also pulses that
+        recur with a shifted RF phase kept and reused; 3 (True) also palindromic runs built from their
+        first half (bmc_plan_create_ex)."""
         L = lib()
         torch_cuda()
         t = np.ascontiguousarray(np.asarray(type_), dtype=np.int32)
@@ -190,9 +192,9 @@ class Plan:
         return dict(n_ops=a.value, n_built=b.value, n_applied=c.value)
 
     def program(self):
-        """(ops [n_ops, 4] int32, row_ptr [O + 1] int32): the compiled row programs."""
+        """(ops [n_ops, 5] int32, row_ptr [O + 1] int32): the compiled row programs."""
         n = self.stats()["n_ops"]
-        ops = np.zeros((max(n, 1), 4), np.int32)
+        ops = np.zeros((max(n, 1), 5), np.int32)
         rp = np.zeros(self.n_offsets + 1, np.int32)
         ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         check(lib().bmc_plan_program(self._h, max(n, 1), ip(ops), ip(rp)), "bmc_plan_program")

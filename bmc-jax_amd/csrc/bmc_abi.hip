@@ -118,6 +118,46 @@ std::vector<PulseInfo> find_pulses(const std::vector<EvKey>& row) {
   return pulses;
 }
 
+// Shortest half worth factorising: the kernel's recombination costs about one chained event.
+constexpr int kMinMirrorHalf = 4;
+// Mirrored RF samples may differ in amplitude by this relative amount (a symmetric shape sampled in
+// floating point, e.g. exp(-((i + 0.5) dt - T / 2)^2 ...), is symmetric to an ulp or two, not bit for
+// bit); the first half's values are used for both. The effect on a state is of the same relative size.
+constexpr double kMirrorAmpTol = 1e-13;
+
+bool near_mirror(const EvKey& a, const EvKey& b) {
+  return a.type == b.type && memcmp(&a.dt, &b.dt, 8) == 0 && memcmp(&a.ph, &b.ph, 8) == 0 &&
+         memcmp(&a.fr, &b.fr, 8) == 0 && fabs(a.amp - b.amp) <= kMirrorAmpTol * fmax(fabs(a.amp), fabs(b.amp));
+}
+
+// The longest even palindromic prefix [e0, e0 + 2h) of the chained run [e0, e0 + len): RF and delay
+// events only, each equal to its mirror image (near_mirror), every RF event with a nonzero amplitude at
+// one phase (bit for bit), which goes to *phase. Returns h, or 0 below kMinMirrorHalf.
+int mirror_half(const std::vector<EvKey>& row, int e0, int len, double* phase) {
+  int lim = 0;  // the prefix of allowed events at one phase
+  bool have_ph = false;
+  double ph = 0.0;
+  for (; lim < len; lim++) {
+    const EvKey& k = row[e0 + lim];
+    if (k.type != BMC_EV_RF && k.type != BMC_EV_DELAY) break;
+    if (k.type == BMC_EV_RF && k.amp != 0.0) {
+      if (have_ph && memcmp(&k.ph, &ph, 8) != 0) break;
+      ph = k.ph;
+      have_ph = true;
+    }
+  }
+  for (int L = lim & ~1; L >= 2 * kMinMirrorHalf; L -= 2) {
+    if (!near_mirror(row[e0 + L - 1], row[e0])) continue;
+    bool pal = true;
+    for (int t = 1; t < L / 2 && pal; t++) pal = near_mirror(row[e0 + t], row[e0 + L - 1 - t]);
+    if (pal) {
+      *phase = ph;
+      return L / 2;
+    }
+  }
+  return 0;
+}
+
 // Greedy run detection for one row: a block [i, i + p) repeated c >= 2 times back to back, with no
 // ADC inside, becomes one BLOCK op. With level >= 2, a pulse that is not part of such a run and
 // recurs later with a shifted phase becomes KEEP (chained, kept), and its later members REUSE ops
@@ -143,9 +183,15 @@ void compile_row(const std::vector<EvKey>& row, int level, std::vector<Op>& ops,
     rot.push_back(c);
     rot.push_back(sn);
   };
+  // with level 3, a chained op whose events open with a palindrome records its half and phase (cos, sin)
+  auto push_chained = [&](Op op) {
+    double ph = 0.0;
+    op.half = level >= 3 ? mirror_half(row, op.e0, op.len, &ph) : 0;
+    push(op, op.half ? cos(ph) : 1.0, op.half ? sin(ph) : 0.0);
+  };
   auto flush_seq = [&](int s0, int s1) {
     if (s1 > s0) {
-      push(Op{OP_SEQ, s0, s1 - s0, 1}, 1.0, 0.0);
+      push(Op{OP_SEQ, s0, s1 - s0, 1, 0}, 1.0, 0.0);
       built += s1 - s0;
       applied += s1 - s0;
     }
@@ -172,7 +218,7 @@ void compile_row(const std::vector<EvKey>& row, int level, std::vector<Op>& ops,
     }
     if (best_c >= 2) {
       flush_seq(seq_start, i);
-      push(Op{OP_BLOCK, i, best_p, best_c}, 1.0, 0.0);
+      push_chained(Op{OP_BLOCK, i, best_p, best_c, 0});
       built += best_p;
       applied += best_c;
       i += best_p * best_c;
@@ -185,14 +231,14 @@ void compile_row(const std::vector<EvKey>& row, int level, std::vector<Op>& ops,
       if (pu.group == kept_group) {
         flush_seq(seq_start, i);
         const double d = row[pu.ref].ph - kept_phase;
-        push(Op{OP_REUSE, pu.a, pu.b - pu.a, 1}, cos(d), sin(d));
+        push(Op{OP_REUSE, pu.a, pu.b - pu.a, 1, 0}, cos(d), sin(d));
         applied += 1;
         i = seq_start = pu.b;
         continue;
       }
       if (pu.has_later) {
         flush_seq(seq_start, i);
-        push(Op{OP_KEEP, pu.a, pu.b - pu.a, 1}, 1.0, 0.0);
+        push_chained(Op{OP_KEEP, pu.a, pu.b - pu.a, 1, 0});
         kept_group = pu.group;
         kept_phase = row[pu.ref].ph;
         built += pu.b - pu.a;
@@ -225,7 +271,7 @@ int bmc_device_count(void) {
 
 int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
                     const double* rf_phase, const double* rf_freq, int reset_after_adc, bmc_plan** out) {
-  return bmc_plan_create_ex(n_offsets, n_events, type, dt, rf_amp, rf_phase, rf_freq, reset_after_adc, 2, out);
+  return bmc_plan_create_ex(n_offsets, n_events, type, dt, rf_amp, rf_phase, rf_freq, reset_after_adc, 3, out);
 }
 
 }  // extern "C"
@@ -237,7 +283,7 @@ namespace {
 int compile_plan(int O, int T, const int32_t* type, const double* dt, const double* rf_amp, const double* rf_phase,
                  const double* rf_freq, int level, std::vector<Ev>* ev, std::vector<Op>& ops, std::vector<double>& rot,
                  std::vector<int32_t>& row_ptr, int64_t& built, int64_t& applied) {
-  if (O <= 0 || T < 0 || (T > 0 && (!type || !dt || !rf_amp || !rf_phase || !rf_freq)) || level < 0 || level > 2) {
+  if (O <= 0 || T < 0 || (T > 0 && (!type || !dt || !rf_amp || !rf_phase || !rf_freq)) || level < 0 || level > 3) {
     set_error("bmc_plan_create: bad arguments");
     return BMC_EINVAL;
   }
@@ -261,7 +307,7 @@ int compile_plan(int O, int T, const int32_t* type, const double* dt, const doub
     if (level > 0) {
       compile_row(row, level, ops, rot, built, applied);
     } else if (T > 0) {
-      ops.push_back(Op{OP_SEQ, 0, T, 1});
+      ops.push_back(Op{OP_SEQ, 0, T, 1, 0});
       rot.push_back(1.0);
       rot.push_back(0.0);
       built += T;
@@ -294,10 +340,11 @@ int bmc_plan_compile(int n_offsets, int n_events, const int32_t* type, const dou
     return BMC_EINVAL;
   }
   for (size_t i = 0; i < ops.size(); i++) {
-    ops_out[4 * i + 0] = ops[i].kind;
-    ops_out[4 * i + 1] = ops[i].e0;
-    ops_out[4 * i + 2] = ops[i].len;
-    ops_out[4 * i + 3] = ops[i].count;
+    ops_out[5 * i + 0] = ops[i].kind;
+    ops_out[5 * i + 1] = ops[i].e0;
+    ops_out[5 * i + 2] = ops[i].len;
+    ops_out[5 * i + 3] = ops[i].count;
+    ops_out[5 * i + 4] = ops[i].half;
     rot_out[2 * i] = rot[2 * i];
     rot_out[2 * i + 1] = rot[2 * i + 1];
   }
@@ -328,6 +375,7 @@ int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const d
   p.reset_after_adc = reset_after_adc ? 1 : 0;
   p.compressed = compress ? 1 : 0;
   p.has_keep = std::any_of(ops.begin(), ops.end(), [](const Op& op) { return op.kind == OP_KEEP; }) ? 1 : 0;
+  p.has_mirror = std::any_of(ops.begin(), ops.end(), [](const Op& op) { return op.half > 0; }) ? 1 : 0;
   // single-event rows: at most one event with dt > 1e-9 (simulate.py:51), a delay or an RF block; no
   // spoiler or gradient anywhere; after that event only zero-length events, any readout being the
   // row's last event (so the row's last_M is the state the event leaves, simulate.py:105-132)
@@ -408,10 +456,11 @@ int bmc_plan_program(const bmc_plan* plan, int64_t capacity, int32_t* ops_out, i
   }
   const Plan& p = plan->p;
   for (size_t i = 0; i < p.h_ops.size(); i++) {
-    ops_out[4 * i + 0] = p.h_ops[i].kind;
-    ops_out[4 * i + 1] = p.h_ops[i].e0;
-    ops_out[4 * i + 2] = p.h_ops[i].len;
-    ops_out[4 * i + 3] = p.h_ops[i].count;
+    ops_out[5 * i + 0] = p.h_ops[i].kind;
+    ops_out[5 * i + 1] = p.h_ops[i].e0;
+    ops_out[5 * i + 2] = p.h_ops[i].len;
+    ops_out[5 * i + 3] = p.h_ops[i].count;
+    ops_out[5 * i + 4] = p.h_ops[i].half;
   }
   for (size_t i = 0; i < p.h_row_ptr.size(); i++) row_ptr_out[i] = p.h_row_ptr[i];
   return BMC_OK;
@@ -452,6 +501,7 @@ int fill_sim_args(const bmc_plan* plan, int V, int nc, int dtype, const void* pa
   a.row_ptr = p.d_row_ptr;
   a.rot = p.d_rot;
   a.keep = p.has_keep;
+  a.mirror = p.has_mirror && !p.has_keep;
   a.O = p.O;
   a.T = p.T;
   a.reset_after_adc = p.reset_after_adc;

@@ -484,6 +484,80 @@ __device__ __forceinline__ void apply_dx_shape(const DxCo& d, const double* v, d
   }
 }
 
+// ---- the palindrome factorisation -------------------------------------------------------------
+// A chained run E_{2h-1} ... E_0 whose events read the same backwards (E_{2h-1-i} = E_i) is Q^R Q with
+// Q = E_{h-1} ... E_0 and Q^R = E_0 ... E_{h-1}. The Bloch-McConnell matrix A of every event is
+// symmetrisable, A = S A^T S^-1 (matrix.py:238-286): at RF phase 0, S = diag(1, f_i, -1, -f_i, 1, f_i, f_mt)
+// over (x, y, z of water and pool i, z of MT) makes exchange (k_i against k_i f_i), precession and RF
+// (antisymmetric, flipped by the y sign) and relaxation (diagonal) agree; at phase phi, S is conjugated by
+// the rotation by phi about z. Hence Q^R's linear part is S Q_A^T S^-1, and its affine column is S w with
+// w the affine column of exp(X'_0) ... exp(X'_{h-1}), X' = [[A^T, S^-1 C], [0, 0]]: a chain through the
+// same h events in reverse order with transposed generators, which the team's spare lane runs beside
+// the others. A pool with f_i = 0 and k_i = 0 (or MT with f = 0 and k = 0) is decoupled and takes s = 1.
+
+// The scale s of one pool's (x, y, z) in S (water: 1): its fraction, or 1 when the pool is decoupled
+// (fraction 0 and rate 0). mirror_ok has ruled out a zero fraction with a nonzero rate.
+__device__ __forceinline__ double sym_scale(double f) { return f > 0.0 ? f : 1.0; }
+
+// The generator of the transposed chain: A^T (precession, RF and pool offsets change sign, exchange
+// k <-> k f swaps) with the affine column S^-1 C (matrix.py:289-293: C = (r1w, r1_i f_i, r1mt f_mt) on
+// the z rows, divided by the pools' scales).
+template <int NC>
+__device__ __forceinline__ void transpose_gen(Gen<NC>& G, const double* prm, double sc) {
+  G.offw = -G.offw;
+  G.w1x = -G.w1x;
+  G.w1y = -G.w1y;
+#pragma unroll
+  for (int i = 0; i < NC; i++) {
+    G.o[i] = -G.o[i];
+    const double k = G.k[i];
+    G.k[i] = G.kw[i];
+    G.kw[i] = k;
+    if (prm[13 + NC + i] > 0.0) G.cc[i] = prm[13 + 3 * NC + i] * sc;
+  }
+  const double kmt = G.kmt;
+  G.kmt = G.kwmt;
+  G.kwmt = kmt;
+  if (prm[4] > 0.0) G.cmt = prm[6] * sc;
+}
+
+// The generator derivative of the transposed chain in shape form: dA^T (the off-diagonal pair of every
+// coupling swapped, antisymmetric terms negated) and d(S^-1 C), which drops the fraction from the
+// C_F / MT_F and C_R1 / MT_R1 terms of a pool with a positive fraction.
+template <int NC>
+__device__ __forceinline__ DxCo dx_transpose(int kind, int ix, DxCo d, const TanCo& t, const double* prm) {
+  double q[7];
+#pragma unroll
+  for (int i = 0; i < 7; i++) q[i] = d.q[i];
+  if (d.shape == DXS_WATER) {
+    d.q[2] = q[4];
+    d.q[4] = q[2];
+    if (prm[4] > 0.0) {
+      if (kind == TK_MT_F) d.q[6] = 0.0;
+      if (kind == TK_MT_R1) d.q[6] = t.c0;
+    }
+  } else if (d.shape == DXS_POOL) {
+    d.q[1] = q[2];
+    d.q[2] = q[1];
+    d.q[4] = -q[4];
+    double f = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+      if (i == ix) f = prm[13 + NC + i];
+    if (f > 0.0) {
+      if (kind == TK_C_F) d.q[6] = 0.0;
+      if (kind == TK_C_R1) d.q[6] = t.c0;
+    }
+  } else if (d.shape == DXS_SHIFT) {
+#pragma unroll
+    for (int i = 0; i <= NC; i++) d.q[i] = -q[i];
+  } else if (d.shape == DXS_B1) {
+    d.q[0] = -q[0];
+    d.q[1] = -q[1];
+  }
+  return d;
+}
+
 // Per-team cache of the MT lineshape, keyed on the event's RF frequency and amplitude presence.
 struct MtCache {
   double fr;
@@ -698,16 +772,21 @@ __device__ __forceinline__ void taylor_terms(const Gen<NC>& G, const DxCo* dx, i
 // UNSWITCH: when every tangent has the same dX shape (and pool), run a copy of the recursion with that
 // shape fixed at compile time, so no term branches on it (for the hottest call site only: each copy is
 // more code).
+// tr_prm (the team's parameters) marks the lane of a palindrome factorisation that runs the transposed
+// chain: its tangent generators are transposed as well (dx_transpose).
 template <int NC, int PT, bool UNSWITCH = false>
 __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
                                        const double* dwp, int m, const double* v0, double vaug0,
                                        const double (*dv0)[3 * (1 + NC) + 1], double* acc,
-                                       double (*dacc)[3 * (1 + NC) + 1]) {
+                                       double (*dacc)[3 * (1 + NC) + 1], const double* tr_prm = nullptr) {
   constexpr int N = 3 * (1 + NC) + 1;
   double v[N], dv[PT > 0 ? PT : 1][N];
   DxCo dx[PT > 0 ? PT : 1];
 #pragma unroll
-  for (int p = 0; p < PT; p++) dx[p] = dx_shape<NC>(kind[p], kidx[p], tc[p], dwp);
+  for (int p = 0; p < PT; p++) {
+    dx[p] = dx_shape<NC>(kind[p], kidx[p], tc[p], dwp);
+    if (tr_prm) dx[p] = dx_transpose<NC>(kind[p], kidx[p], dx[p], tc[p], tr_prm);
+  }
 #pragma unroll
   for (int i = 0; i < N; i++) v[i] = v0[i];
 #pragma unroll
@@ -880,6 +959,228 @@ __device__ __forceinline__ void rotate_xy(double* M, double (*dM)[3 * (1 + NC) +
   }
 }
 
+// Whether the team's parameters and tangents admit the palindrome factorisation: every pool (and MT)
+// either has a positive fraction or is decoupled (fraction 0, rate 0) with no tangent along its rate.
+template <int NC, int PT>
+__device__ __forceinline__ bool mirror_ok(const double* prm, const int* kind, const int* kidx) {
+  constexpr int PTA = PT > 0 ? PT : 1;
+  bool ok = prm[4] > 0.0 || (prm[4] == 0.0 && prm[3] == 0.0);
+#pragma unroll
+  for (int p = 0; p < PTA; p++) ok = ok && !(p < PT && kind[p] == TK_MT_K && !(prm[4] > 0.0));
+#pragma unroll
+  for (int i = 0; i < NC; i++) {
+    const double f = prm[13 + NC + i];
+    ok = ok && (f > 0.0 || (f == 0.0 && prm[13 + i] == 0.0));
+#pragma unroll
+    for (int p = 0; p < PTA; p++) ok = ok && !(p < PT && kind[p] == TK_C_K && kidx[p] == i && !(f > 0.0));
+  }
+  return ok;
+}
+
+// y = S^e x for S of the palindrome factorisation (e = +1 or -1) at RF phase (c, s) = (cos phi, sin phi):
+// S = R(phi) S0 R(-phi), S0 = diag(1, s_i, -1, -s_i, 1, s_i, s_mt).
+template <int NC>
+__device__ __forceinline__ void apply_sym(const double* prm, double c, double s, bool inverse, const double* x,
+                                          double* y) {
+  constexpr int NT = 1 + NC, N = 3 * NT + 1;
+#pragma unroll
+  for (int i = 0; i < N; i++) y[i] = x[i];
+#pragma unroll
+  for (int i = 0; i < NT; i++) {  // R(-phi)
+    const double a = y[i], b = y[NT + i];
+    y[i] = fma(c, a, s * b);
+    y[NT + i] = fma(-s, a, c * b);
+  }
+#pragma unroll
+  for (int i = 0; i < NT; i++) {
+    const double sc = i == 0 ? 1.0 : sym_scale(prm[13 + NC + i - 1]);
+    const double m = inverse ? 1.0 / sc : sc;
+    y[i] *= m;
+    y[NT + i] *= -m;
+    y[2 * NT + i] *= m;
+  }
+  {
+    const double sc = sym_scale(prm[4]);
+    y[3 * NT] *= inverse ? 1.0 / sc : sc;
+  }
+#pragma unroll
+  for (int i = 0; i < NT; i++) {  // R(phi)
+    const double a = y[i], b = y[NT + i];
+    y[i] = fma(c, a, -s * b);
+    y[NT + i] = fma(s, a, c * b);
+  }
+}
+
+// y += g (.) x for g = S^-1 dS of one tangent: 1 / f on the (x, y, z) of the pool whose fraction the
+// tangent moves (C_F), 1 / f_mt on MT's z (MT_F), when that fraction is positive; no other tangent
+// moves S.
+template <int NC>
+__device__ __forceinline__ void add_dsym(const double* prm, int kind, int ix, const double* x, double* y) {
+  constexpr int NT = 1 + NC;
+  if (kind == TK_MT_F && prm[4] > 0.0) y[3 * NT] = fma(1.0 / prm[4], x[3 * NT], y[3 * NT]);
+  if (kind == TK_C_F) {
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+      const double f = prm[13 + NC + i];
+      if (i == ix && f > 0.0) {
+        const double g = 1.0 / f;
+        y[1 + i] = fma(g, x[1 + i], y[1 + i]);
+        y[NT + 1 + i] = fma(g, x[NT + 1 + i], y[NT + 1 + i]);
+        y[2 * NT + 1 + i] = fma(g, x[2 * NT + 1 + i], y[2 * NT + 1 + i]);
+      }
+    }
+  }
+}
+
+// Lanes of one team exchange values through LDS: no lane reads before every write issued earlier in
+// program order has landed (a team never spans two waves, whose LDS accesses complete in order).
+__device__ __forceinline__ void team_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The propagator of the palindromic run [e0, e0 + 2h) (with tangents) by its factorisation Q^R Q: lanes
+// 0..N chain the columns of Q over events e0 .. e0 + h - 1 as a chained block does, while lane N + 1
+// (spare in every team wider than the augmented size) chains the transposed generators over the same
+// events in reverse order from the affine unit column, giving w. Then, per lane j <= N,
+//   v = S^-1 Q[:, j], dv = S^-1 dQ[:, j] - g v, r = Q_A^T v, dr = Q_A^T dv + dQ_A^T v,
+//   Pi[:, j] = S (r + [j = N] w), dPi[:, j] = S (g (r + [j = N] w) + dr + [j = N] dw)
+// with g = S^-1 dS. Q_A's columns (then each tangent's dQ_A) and w pass through `stage`, the team's
+// LDS area of (N + 1) N doubles. Returns false, with Pi and dPi unspecified, when an event needs
+// squarings on some lane of the team (its propagator cannot be chained column by column); the caller
+// then chains the run event by event.
+// It is a call of its own (not inlined), with its arguments and result passed by value, so that its
+// register allocation does not raise the pressure in the caller's event loops.
+template <int NC, int PT>
+struct MirrorIn {  // a pass's tangents (kinds, pools) and the pools' offsets
+  int kind[PT > 0 ? PT : 1], kidx[PT > 0 ? PT : 1];
+  double dwp[NC > 0 ? NC : 1];
+};
+template <int NC, int PT>
+struct MirrorOut {  // the run's propagator column and its tangents; ok = false: chain event by event
+  double Pi[3 * (1 + NC) + 1];
+  double dPi[PT > 0 ? PT : 1][3 * (1 + NC) + 1];
+  bool ok;
+};
+
+template <int NC, int PT, int TEAM>
+__device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, MirrorIn<NC, PT> in, const Ev* evr, int e0,
+                                                       int h, double cph, double sph, double nan_l1, int lane,
+                                                       double* stage) {
+  constexpr int NT = 1 + NC, N = 3 * NT + 1, NA = N + 1;
+  constexpr int PTA = PT > 0 ? PT : 1;
+  static_assert(TEAM > NA, "the factorisation needs a spare lane");
+  MirrorOut<NC, PT> out;
+  out.ok = false;
+  double* Pi = out.Pi;
+  double (*dPi)[N] = out.dPi;
+  const int* kind = in.kind;
+  const int* kidx = in.kidx;
+  const double* dwp = in.dwp;
+  MtCache mc;
+  mc.fr = __builtin_nan("");
+  mc.s = MtShape{0.0, 0.0, 0.0, 0.0};
+  const bool rev = lane == NA;
+  const double vaug = (lane == N || rev) ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    Pi[i] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+    for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
+  }
+  bool bad = false;
+  int ei = rev ? e0 + h - 1 : e0;
+  const int step = rev ? -1 : 1;
+  Ev e_next = evr[ei];
+  for (int t = 0; t < h; t++) {
+    const Ev e = e_next;
+    ei += step;
+    if (t + 1 < h) e_next = evr[ei];
+    if (e.dt > 1e-9) {
+      Gen<NC> G;
+      TanCo tc[PTA];
+      double l1, sc;
+      int m, s;
+      bool isn;
+      build_gen<NC, PT>(prm, e, mc, kind, kidx, G, tc, l1, sc, nan_l1, m, s, isn);
+      bad = bad || s > 0 || isn;
+      if (!bad) {
+        if (rev) transpose_gen<NC>(G, prm, sc);
+        taylor<NC, PT, true>(G, tc, kind, kidx, dwp, m, Pi, vaug, dPi, Pi, dPi, rev ? prm : nullptr);
+      }
+    }
+  }
+  if (team_sum<TEAM>(bad ? 1.0 : 0.0) != 0.0) return out;
+
+  // recombination: lanes < N stage their column of Q_A (or dQ_A), the reversed lane w (or dw) as column N
+  const int slot = lane < N ? lane : (rev ? N : -1);
+  double v[N], dv[PTA][N], r[N], dr[PTA][N];
+  apply_sym<NC>(prm, cph, sph, true, Pi, v);
+#pragma unroll
+  for (int p = 0; p < PTA; p++) {
+    apply_sym<NC>(prm, cph, sph, true, dPi[p], dv[p]);
+    double gv[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) gv[i] = 0.0;
+    if (p < PT) add_dsym<NC>(prm, kind[p], kidx[p], v, gv);
+#pragma unroll
+    for (int i = 0; i < N; i++) dv[p][i] -= gv[i];
+  }
+  const bool aff = lane == N;
+  if (slot >= 0) {
+#pragma unroll
+    for (int b = 0; b < N; b++) stage[slot * N + b] = Pi[b];
+  }
+  team_lds_sync();
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    double ra = 0.0, dra[PTA];
+#pragma unroll
+    for (int p = 0; p < PTA; p++) dra[p] = 0.0;
+#pragma unroll
+    for (int b = 0; b < N; b++) {
+      const double qba = stage[a * N + b];
+      ra = fma(qba, v[b], ra);
+#pragma unroll
+      for (int p = 0; p < PT; p++) dra[p] = fma(qba, dv[p][b], dra[p]);
+    }
+    r[a] = aff ? ra + stage[N * N + a] : ra;
+#pragma unroll
+    for (int p = 0; p < PTA; p++) dr[p][a] = dra[p];
+  }
+#pragma unroll
+  for (int p = 0; p < PT; p++) {
+    team_lds_sync();
+    if (slot >= 0) {
+#pragma unroll
+      for (int b = 0; b < N; b++) stage[slot * N + b] = dPi[p][b];
+    }
+    team_lds_sync();
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+      double da = 0.0;
+#pragma unroll
+      for (int b = 0; b < N; b++) da = fma(stage[a * N + b], v[b], da);
+      dr[p][a] += aff ? da + stage[N * N + a] : da;
+    }
+  }
+  team_lds_sync();  // the stage is free again once every read of it has landed
+  const bool own = lane <= N;
+  apply_sym<NC>(prm, cph, sph, false, r, v);
+#pragma unroll
+  for (int i = 0; i < N; i++) Pi[i] = own ? v[i] : 0.0;
+#pragma unroll
+  for (int p = 0; p < PTA; p++) {
+    if (p < PT) add_dsym<NC>(prm, kind[p], kidx[p], r, dr[p]);
+    apply_sym<NC>(prm, cph, sph, false, dr[p], v);
+#pragma unroll
+    for (int i = 0; i < N; i++) dPi[p][i] = (own && p < PT) ? v[i] : 0.0;
+  }
+  out.ok = true;
+  return out;
+}
+
 // Runs one offset row's program (the event scan, simulate.py:80-132) on the team's state M and its
 // tangents dM: SEQ ops event by event, BLOCK ops chained into one propagator applied `count` times.
 // prm / st0: the team's parameters and start state (+ seeds) in LDS; st_park: LDS room for the state
@@ -888,12 +1189,15 @@ __device__ __forceinline__ void rotate_xy(double* M, double (*dM)[3 * (1 + NC) +
 // LDS slot, (1 + PT) N (N + 1) doubles, lane j's column at stride N + 1) and REUSE ops apply it
 // conjugated by the op's rotation (rot[2 op], rot[2 op + 1]) = (cos d, sin d). Without it a KEEP op is
 // a BLOCK and the kept-pulse code is not compiled in: it would cost the other plans registers.
-template <int NC, int PT, int TEAM, bool KEEP = false>
+// With MIR, mir is the team's LDS stage of (N + 1) N doubles for the palindrome factorisation of chained
+// ops with a half (mirror_chain), in teams with a spare lane; without it those ops chain every event.
+template <int NC, int PT, int TEAM, bool KEEP = false, bool MIR = false>
 __device__ __forceinline__ void run_row(const double* prm, const double* st0, double* st_park, const int* kind,
                                         const int* kidx, const double* dwp, const Ev* ev_row, const Op* ops, int op0,
                                         int op1, int reset_after_adc, double nan_l1, int lane,
                                         double* M, double (*dM)[3 * (1 + NC) + 1], void* traj, long traj_base,
-                                        int traj_dtype, const double* rot = nullptr, double* keep = nullptr) {
+                                        int traj_dtype, const double* rot = nullptr, double* keep = nullptr,
+                                        double* mir = nullptr) {
   constexpr int NT = 1 + NC, N = 3 * NT + 1, NA = N + 1;
   constexpr int PTA = PT > 0 ? PT : 1;
   auto load_start = [&]() {
@@ -987,14 +1291,42 @@ __device__ __forceinline__ void run_row(const double* prm, const double* st0, do
         }
       }
       double Pi[N], dPi[PTA][N];
+      int ei0 = op.e0;  // the first event chained one by one below
+      bool mirrored = false;
+      if constexpr (MIR && TEAM > NA) {  // a palindromic opening is built from its first half (mirror_chain)
+        if (op.half > 0 && rot && mir && mirror_ok<NC, PT>(prm, kind, kidx)) {
+          MirrorIn<NC, PT> in;
 #pragma unroll
-      for (int i = 0; i < N; i++) {
-        Pi[i] = (i == lane) ? 1.0 : 0.0;
+          for (int p = 0; p < PTA; p++) {
+            in.kind[p] = kind[p];
+            in.kidx[p] = kidx[p];
+          }
 #pragma unroll
-        for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
+          for (int i = 0; i < (NC > 0 ? NC : 1); i++) in.dwp[i] = NC > 0 ? dwp[i] : 0.0;
+          const MirrorOut<NC, PT> mo =
+              mirror_chain<NC, PT, TEAM>(prm, in, evr, op.e0, op.half, rot[2 * oi], rot[2 * oi + 1], nan_l1, lane, mir);
+          mirrored = mo.ok;
+          if (mirrored) {
+            ei0 = op.e0 + 2 * op.half;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+              Pi[i] = mo.Pi[i];
+#pragma unroll
+              for (int p = 0; p < PTA; p++) dPi[p][i] = mo.dPi[p][i];
+            }
+          }
+        }
       }
-      Ev e_next = evr[op.e0];
-      for (int ei = op.e0; ei < op.e0 + op.len; ei++) {
+      if (!mirrored) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          Pi[i] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+          for (int p = 0; p < PTA; p++) dPi[p][i] = 0.0;
+        }
+      }
+      Ev e_next = evr[min(ei0, op.e0 + op.len - 1)];
+      for (int ei = ei0; ei < op.e0 + op.len; ei++) {
         const Ev e = e_next;
         if (ei + 1 < op.e0 + op.len) e_next = evr[ei + 1];  // the next record is in flight meanwhile
         if (e.dt > 1e-9) {

@@ -27,8 +27,13 @@ struct Ev {
 //                   (every sample's phase shifted by d): their propagator is R(d) P R(d)^T, with P the
 //                   kept one and R(d) the rotation by d about z of every pool's (x, y). Applied once
 //                   as M <- R(d) P R(-d) M; the op's (cos d, sin d) is in the plan's rotation table.
+// `half` (BLOCK and KEEP ops): when h > 0, events [e0, e0 + 2h) read the same backwards as forwards
+// (bit for bit), are RF or delay events only, and every RF event among them with a nonzero amplitude
+// has the same phase phi, whose (cos phi, sin phi) is the op's entry in the rotation table. Kernels
+// with a spare lane per team may then chain only the first h events (the run's palindrome
+// factorisation); the others chain all of them as usual.
 struct Op {
-  int32_t kind, e0, len, count;
+  int32_t kind, e0, len, count, half;
 };
 
 enum { OP_SEQ = 0, OP_BLOCK = 1, OP_KEEP = 2, OP_REUSE = 3 };
@@ -91,6 +96,7 @@ struct SimArgs {
   int t_off;              // first tangent of this launch's pass 0 (a launch may carry the last pass alone)
   const double* rot;      // [n_ops, 2] (cos d, sin d) of REUSE ops
   int keep;               // the plan holds KEEP ops: the launch gives every team an LDS slot for one propagator
+  int mirror;             // the plan holds chained ops with a palindromic half (and no KEEP ops)
 };
 
 struct Plan {
@@ -103,6 +109,7 @@ struct Plan {
   int64_t n_ops = 0, n_built = 0, n_applied = 0;
   int compressed = 1;
   int has_keep = 0;
+  int has_mirror = 0;  // some BLOCK / KEEP op has a palindromic half
   int cw_rows = 0;  // every row propagates at most one event (a CW block) and ends at its readout
   std::vector<Op> h_ops;          // host copy of the program
   std::vector<int32_t> h_row_ptr; // host copy of the row offsets

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define BMC_ABI_VERSION 2
+#define BMC_ABI_VERSION 3
 
 #define BMC_F32 0
 #define BMC_F64 1
@@ -79,14 +79,17 @@ int bmc_plan_create(int n_offsets, int n_events, const int32_t* type, const doub
                     const double* rf_phase, const double* rf_freq, int reset_after_adc, bmc_plan** out);
 /* As bmc_plan_create with a compression level: 0 keeps every event in order as its own step (no chained
  * runs), which bmc_simulate_trajectory needs; 1 chains runs of identical events repeated back to back;
- * 2 (bmc_plan_create's level) also keeps a pulse that recurs later in its row with every sample's RF
- * phase shifted by one constant d (a scanner train's accumulated phase, sequences/pulseq.py:60-66)
- * and applies it again rotated by d about z instead of rebuilding it. */
+ * 2 also keeps a pulse that recurs later in its row with every sample's RF phase shifted by one
+ * constant d (a scanner train's accumulated phase, sequences/pulseq.py:60-66) and applies it again
+ * rotated by d about z instead of rebuilding it; 3 (bmc_plan_create's level) also marks chained runs
+ * that open with a palindrome (a symmetric pulse shape at one RF phase: mirrored samples equal in type,
+ * dt, phase and frequency and in amplitude to 1e-13 relative), whose propagator the kernels build from
+ * the first half (the same result to rounding). */
 int bmc_plan_create_ex(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
                        const double* rf_phase, const double* rf_freq, int reset_after_adc, int compress,
                        bmc_plan** out);
 /* The row programs bmc_plan_create_ex would build, on the host only (no device call): ops_out
- * [n_ops, 4] and rot_out [n_ops, 2] as bmc_plan_program describes them, row_ptr_out [n_offsets + 1].
+ * [n_ops, 5] and rot_out [n_ops, 2] as bmc_plan_program describes them, row_ptr_out [n_offsets + 1].
  * With all three outputs NULL it only stores the op count in n_ops_out (a size query). Replaces
  * nothing in the reference; it lets a caller inspect a compilation without a GPU. */
 int bmc_plan_compile(int n_offsets, int n_events, const int32_t* type, const double* dt, const double* rf_amp,
@@ -95,10 +98,13 @@ int bmc_plan_compile(int n_offsets, int n_events, const int32_t* type, const dou
 int bmc_plan_destroy(bmc_plan* plan);
 /* Statistics of a plan: ops and the number of events whose propagator is built per offset row. */
 int bmc_plan_stats(const bmc_plan* plan, int64_t* n_ops, int64_t* n_built, int64_t* n_applied);
-/* The row programs: ops_out [n_ops, 4] = (kind, first event, length, repeat count), row_ptr_out
+/* The row programs: ops_out [n_ops, 5] = (kind, first event, length, repeat count, half), row_ptr_out
  * [n_offsets + 1]. Kinds: 0 SEQ (events one by one), 1 BLOCK (chained once, applied `count` times),
  * 2 KEEP (as BLOCK, and kept), 3 REUSE (the kept block rotated by the op's phase offset d: (cos d,
- * sin d) in bmc_plan_compile's rot_out). capacity is the number of ops ops_out can hold. */
+ * sin d) in bmc_plan_compile's rot_out). half h > 0 on a BLOCK or KEEP op: its first 2h events read
+ * the same backwards (RF and delay events, one RF phase phi, whose (cos phi, sin phi) is the op's
+ * rot_out entry), so the kernel builds their propagator from the first h (palindrome factorisation).
+ * capacity is the number of ops ops_out can hold. */
 int bmc_plan_program(const bmc_plan* plan, int64_t capacity, int32_t* ops_out, int32_t* row_ptr_out);
 
 /*

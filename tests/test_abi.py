@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(_native.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, f"not exported: {missing}"
-    assert lib.bmc_abi_version() == 2
+    assert lib.bmc_abi_version() == 3
 
 
 def test_errors_are_reported_without_a_device():

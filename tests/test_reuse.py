@@ -145,3 +145,38 @@ def test_the_reference_caches_compile_to_kept_pulses():
             n_pulses = len(_pulse_starts(np.asarray(ev.type[o])))
             assert np.sum(row[:, 0] == OP_KEEP) == 1, name
             assert np.sum(row[:, 0] == OP_REUSE) == n_pulses - 1, name
+
+
+def _sym_train(ppm, phase=0.0):
+    """Three Gaussian pulses of 64 symmetric samples with gaps, RF at `phase` (oracle's pulsed_train)."""
+    ev = dict(O.pulsed_train(np.asarray(ppm, float), b1_peak_uT=2.0, n_pulses=3, pulse_s=0.02, n_samples=64,
+                             gap_s=0.02, recovery_s=1.0, b0_T=3.0, dtype=np.float64))
+    ev["rf_phase"] = np.where(ev["type"] == 1, phase, 0.0)
+    return ev
+
+
+def test_plan_marks_only_palindromes():
+    """Host compilation: the half of a symmetric pulse at one phase, none for a phase that changes inside
+    the pulse or a pulse that is not symmetric; level 2 marks nothing."""
+    from bmc_jax import _native
+    ev = _sym_train([1.0, 2.0])
+    ops, rot, rp = _native.compile_program(ev["type"], ev["dt"], ev["rf_amp"], ev["rf_phase"], ev["rf_freq"], 3)
+    blk = ops[ops[:, 0] == 1]
+    assert np.all(blk[:, 4] == 32) and np.allclose(rot[ops[:, 0] == 1], [1.0, 0.0])
+    ev7 = _sym_train([1.0], phase=0.7)
+    ops, rot, _ = _native.compile_program(ev7["type"], ev7["dt"], ev7["rf_amp"], ev7["rf_phase"], ev7["rf_freq"], 3)
+    assert np.allclose(rot[ops[:, 4] > 0], [np.cos(0.7), np.sin(0.7)])
+    ops2, _, _ = _native.compile_program(ev["type"], ev["dt"], ev["rf_amp"], ev["rf_phase"], ev["rf_freq"], 2)
+    assert np.all(ops2[:, 4] == 0)
+    chirp = dict(ev7)
+    chirp["rf_phase"] = np.where(ev7["type"] == 1, np.cumsum(ev7["type"] == 1, axis=1) * 1e-3, 0.0)
+    ops, _, _ = _native.compile_program(chirp["type"], chirp["dt"], chirp["rf_amp"], chirp["rf_phase"],
+                                        chirp["rf_freq"], 3)
+    assert np.all(ops[:, 4] == 0)
+    skew = dict(ev)
+    amp = skew["rf_amp"].copy()
+    amp[:, [2, 67, 132]] *= 1.0 + 1e-12      # the second sample of every pulse: still repeated, not symmetric
+    skew["rf_amp"] = amp
+    ops, _, _ = _native.compile_program(skew["type"], skew["dt"], skew["rf_amp"], skew["rf_phase"],
+                                        skew["rf_freq"], 3)
+    assert np.any(ops[:, 0] == 1) and np.all(ops[:, 4] == 0)
