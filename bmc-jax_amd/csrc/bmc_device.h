@@ -1073,11 +1073,17 @@ __device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, Mirror
   static_assert(TEAM > NA, "the factorisation needs a spare lane");
   MirrorOut<NC, PT> out;
   out.ok = false;
-  double* Pi = out.Pi;
-  double (*dPi)[N] = out.dPi;
-  const int* kind = in.kind;
-  const int* kidx = in.kidx;
-  const double* dwp = in.dwp;
+  // the chain works on local copies: the by-value argument and the returned record live in scratch
+  // memory, and working in them directly sent every event's series through it
+  double Pi[N], dPi[PTA][N], dwp[NC > 0 ? NC : 1];
+  int kind[PTA], kidx[PTA];
+#pragma unroll
+  for (int p = 0; p < PTA; p++) {
+    kind[p] = in.kind[p];
+    kidx[p] = in.kidx[p];
+  }
+#pragma unroll
+  for (int i = 0; i < (NC > 0 ? NC : 1); i++) dwp[i] = in.dwp[i];
   MtCache mc;
   mc.fr = __builtin_nan("");
   mc.s = MtShape{0.0, 0.0, 0.0, 0.0};
@@ -1169,13 +1175,13 @@ __device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, Mirror
   const bool own = lane <= N;
   apply_sym<NC>(prm, cph, sph, false, r, v);
 #pragma unroll
-  for (int i = 0; i < N; i++) Pi[i] = own ? v[i] : 0.0;
+  for (int i = 0; i < N; i++) out.Pi[i] = own ? v[i] : 0.0;
 #pragma unroll
   for (int p = 0; p < PTA; p++) {
     if (p < PT) add_dsym<NC>(prm, kind[p], kidx[p], r, dr[p]);
     apply_sym<NC>(prm, cph, sph, false, dr[p], v);
 #pragma unroll
-    for (int i = 0; i < N; i++) dPi[p][i] = (own && p < PT) ? v[i] : 0.0;
+    for (int i = 0; i < N; i++) out.dPi[p][i] = (own && p < PT) ? v[i] : 0.0;
   }
   out.ok = true;
   return out;
