@@ -774,11 +774,13 @@ __device__ __forceinline__ void taylor_terms(const Gen<NC>& G, const DxCo* dx, i
 // more code).
 // tr_prm (the team's parameters) marks the lane of a palindrome factorisation that runs the transposed
 // chain: its tangent generators are transposed as well (dx_transpose).
-template <int NC, int PT, bool UNSWITCH = false>
+// dv0 is either an array of PT tangent columns or nullptr (zero tangents), told apart by its type: a runtime
+// test of a local array's address against null cannot be folded (0 is a valid private address), and it
+// kept the caller's array in scratch memory.
+template <int NC, int PT, bool UNSWITCH = false, typename DV0>
 __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
-                                       const double* dwp, int m, const double* v0, double vaug0,
-                                       const double (*dv0)[3 * (1 + NC) + 1], double* acc,
-                                       double (*dacc)[3 * (1 + NC) + 1], const double* tr_prm = nullptr) {
+                                       const double* dwp, int m, const double* v0, double vaug0, DV0 dv0,
+                                       double* acc, double (*dacc)[3 * (1 + NC) + 1], const double* tr_prm = nullptr) {
   constexpr int N = 3 * (1 + NC) + 1;
   double v[N], dv[PT > 0 ? PT : 1][N];
   DxCo dx[PT > 0 ? PT : 1];
@@ -792,7 +794,12 @@ __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const 
 #pragma unroll
   for (int p = 0; p < PT; p++)
 #pragma unroll
-    for (int i = 0; i < N; i++) dv[p][i] = dv0 ? dv0[p][i] : 0.0;
+    for (int i = 0; i < N; i++) {
+      if constexpr (std::is_same_v<DV0, std::nullptr_t>)
+        dv[p][i] = 0.0;
+      else
+        dv[p][i] = dv0[p][i];
+    }
 #pragma unroll
   for (int i = 0; i < N; i++) acc[i] = v[i];
 #pragma unroll
@@ -1014,21 +1021,24 @@ __device__ __forceinline__ void apply_sym(const double* prm, double c, double s,
 // y += g (.) x for g = S^-1 dS of one tangent: 1 / f on the (x, y, z) of the pool whose fraction the
 // tangent moves (C_F), 1 / f_mt on MT's z (MT_F), when that fraction is positive; no other tangent
 // moves S.
+// Branch-free, every entry at a compile-time index (a zero g where the tangent does not move S): stores
+// under a runtime condition were merged into one store at a runtime index, which sent the arrays it
+// writes to scratch memory.
 template <int NC>
 __device__ __forceinline__ void add_dsym(const double* prm, int kind, int ix, const double* x, double* y) {
   constexpr int NT = 1 + NC;
-  if (kind == TK_MT_F && prm[4] > 0.0) y[3 * NT] = fma(1.0 / prm[4], x[3 * NT], y[3 * NT]);
-  if (kind == TK_C_F) {
+  {
+    const double f = prm[4];
+    const double g = (kind == TK_MT_F && f > 0.0) ? 1.0 / f : 0.0;
+    y[3 * NT] = fma(g, x[3 * NT], y[3 * NT]);
+  }
 #pragma unroll
-    for (int i = 0; i < NC; i++) {
-      const double f = prm[13 + NC + i];
-      if (i == ix && f > 0.0) {
-        const double g = 1.0 / f;
-        y[1 + i] = fma(g, x[1 + i], y[1 + i]);
-        y[NT + 1 + i] = fma(g, x[NT + 1 + i], y[NT + 1 + i]);
-        y[2 * NT + 1 + i] = fma(g, x[2 * NT + 1 + i], y[2 * NT + 1 + i]);
-      }
-    }
+  for (int i = 0; i < NC; i++) {
+    const double f = prm[13 + NC + i];
+    const double g = (kind == TK_C_F && i == ix && f > 0.0) ? 1.0 / f : 0.0;
+    y[1 + i] = fma(g, x[1 + i], y[1 + i]);
+    y[NT + 1 + i] = fma(g, x[NT + 1 + i], y[NT + 1 + i]);
+    y[2 * NT + 1 + i] = fma(g, x[2 * NT + 1 + i], y[2 * NT + 1 + i]);
   }
 }
 
@@ -1121,6 +1131,9 @@ __device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, Mirror
 
   // recombination: lanes < N stage their column of Q_A (or dQ_A), the reversed lane w (or dw) as column N
   const int slot = lane < N ? lane : (rev ? N : -1);
+  // the columns go to the stage by volatile stores: a plain copy loop became a memcpy from the local
+  // arrays, which kept them in scratch memory through the whole chain
+  volatile double* stage_w = stage;
   double v[N], dv[PTA][N], r[N], dr[PTA][N];
   apply_sym<NC>(prm, cph, sph, true, Pi, v);
 #pragma unroll
@@ -1136,7 +1149,7 @@ __device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, Mirror
   const bool aff = lane == N;
   if (slot >= 0) {
 #pragma unroll
-    for (int b = 0; b < N; b++) stage[slot * N + b] = Pi[b];
+    for (int b = 0; b < N; b++) stage_w[slot * N + b] = Pi[b];
   }
   team_lds_sync();
 #pragma unroll
@@ -1155,12 +1168,14 @@ __device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, Mirror
 #pragma unroll
     for (int p = 0; p < PTA; p++) dr[p][a] = dra[p];
   }
-#pragma unroll
-  for (int p = 0; p < PT; p++) {
+  // the tangent index is a compile-time constant (a loop holding the LDS syncs was left rolled, and
+  // indexing dPi / dr by a runtime p kept them in scratch memory)
+  static_for<0, PT>([&](auto pc) {
+    constexpr int p = decltype(pc)::value;
     team_lds_sync();
     if (slot >= 0) {
 #pragma unroll
-      for (int b = 0; b < N; b++) stage[slot * N + b] = dPi[p][b];
+      for (int b = 0; b < N; b++) stage_w[slot * N + b] = dPi[p][b];
     }
     team_lds_sync();
 #pragma unroll
@@ -1170,7 +1185,7 @@ __device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, Mirror
       for (int b = 0; b < N; b++) da = fma(stage[a * N + b], v[b], da);
       dr[p][a] += aff ? da + stage[N * N + a] : da;
     }
-  }
+  });
   team_lds_sync();  // the stage is free again once every read of it has landed
   const bool own = lane <= N;
   apply_sym<NC>(prm, cph, sph, false, r, v);
