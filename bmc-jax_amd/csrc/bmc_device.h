@@ -52,6 +52,16 @@ __host__ __device__ constexpr int teams_per_block() {
   return TEAM == 12 ? 20 : 256 / TEAM;  // 256-thread blocks
 }
 template <int TEAM>
+__host__ __device__ constexpr int teams_per_wave() {
+  return TEAM == 12 ? 5 : 64 / TEAM;
+}
+// Teams of one tangent pass, rounded up to whole waves: numbered this way, every wave's teams share one
+// pass, so its tangent kinds are wave-uniform (scalar branches in the event setup).
+template <int TEAM>
+__host__ __device__ constexpr long pass_teams(long rows) {
+  return (rows + teams_per_wave<TEAM>() - 1) / teams_per_wave<TEAM>() * teams_per_wave<TEAM>();
+}
+template <int TEAM>
 __device__ __forceinline__ int team_base_in_wave() {  // the team's first lane within its wave
   if constexpr (TEAM == 12) {
     const int wl = threadIdx.x & 63;
@@ -1075,12 +1085,18 @@ struct MirrorOut {  // the run's propagator column and its tangents; ok = false:
 };
 
 template <int NC, int PT, int TEAM>
-__device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, MirrorIn<NC, PT> in, const Ev* evr, int e0,
+__device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm_g, MirrorIn<NC, PT> in, const Ev* evr, int e0,
                                                        int h, double cph, double sph, double nan_l1, int lane,
-                                                       double* stage) {
+                                                       double* stage_g) {
   constexpr int NT = 1 + NC, N = 3 * NT + 1, NA = N + 1;
   constexpr int PTA = PT > 0 ? PT : 1;
   static_assert(TEAM > NA, "the factorisation needs a spare lane");
+  // prm and stage are the team's LDS areas: a call of its own receives them as generic pointers, whose
+  // loads are flat loads; told that they point into LDS, the compiler emits LDS loads
+  const double* prm = prm_g;
+  double* stage = stage_g;
+  __builtin_assume(__builtin_amdgcn_is_shared((const __attribute__((address_space(0))) void*)prm));
+  __builtin_assume(__builtin_amdgcn_is_shared((const __attribute__((address_space(0))) void*)stage));
   MirrorOut<NC, PT> out;
   out.ok = false;
   // the chain works on local copies: the by-value argument and the returned record live in scratch
@@ -1088,9 +1104,9 @@ __device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm, Mirror
   double Pi[N], dPi[PTA][N], dwp[NC > 0 ? NC : 1];
   int kind[PTA], kidx[PTA];
 #pragma unroll
-  for (int p = 0; p < PTA; p++) {
-    kind[p] = in.kind[p];
-    kidx[p] = in.kidx[p];
+  for (int p = 0; p < PTA; p++) {  // the caller's tangents are wave-uniform (pass_teams)
+    kind[p] = __builtin_amdgcn_readfirstlane(in.kind[p]);
+    kidx[p] = __builtin_amdgcn_readfirstlane(in.kidx[p]);
   }
 #pragma unroll
   for (int i = 0; i < (NC > 0 ? NC : 1); i++) dwp[i] = in.dwp[i];

@@ -55,7 +55,8 @@ __host__ __device__ constexpr int mirror_stage_doubles() {
 }
 
 // One team's (voxel, offset row, pass) of a launch with `total` teams over n_vox voxels (voxel
-// index through a.vlist when a compacted list is given). Every thread of the block calls it: it
+// index through a.vlist when a compacted list is given). Each pass holds pass_teams(n_vox O) teams,
+// the last few of them idle, so that no wave spans two passes. Every thread of the block calls it: it
 // holds block barriers before its first early exit.
 template <int NC, int PT, int TEAM, bool LISTED, bool KEEP, bool MIR>
 __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total, int n_vox,
@@ -68,13 +69,19 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   constexpr int PTA = PT > 0 ? PT : 1;
   const int lane = team_lane<TEAM>();
   const int tib = team_index<TEAM>();
+  const long rows = (long)n_vox * a.O;
+  const long per_pass = pass_teams<TEAM>(rows);
   bool dead = team >= total;
   const long tt = dead ? 0 : team;
-  const int o = (int)(tt % a.O);
-  const long t2 = tt / a.O;
-  const int vi = (int)(t2 % n_vox);
+  const int q = (int)(tt / per_pass);
+  long r = tt % per_pass;
+  if (r >= rows) {  // the padding that rounds a pass up to whole waves
+    dead = true;
+    r = 0;
+  }
+  const int o = (int)(r % a.O);
+  const int vi = (int)(r / a.O);
   const int v = LISTED ? a.vlist[vi] : vi;
-  const int q = (int)(t2 / n_vox);
   if (!dead && a.active && !a.active[v]) dead = true;
   // a block with no live team leaves before touching its records (stopped LM problems)
   if (a.active && !__syncthreads_or(!dead)) return;
@@ -123,13 +130,14 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
     if (live == 0) return;
   }
 
-  // tangents of this pass
+  // tangents of this pass: the same on every lane of the wave (no wave spans two passes), so they are
+  // read into scalar registers and the event setup branches on them as a wave
   int kind[PTA], kidx[PTA];
 #pragma unroll
   for (int p = 0; p < PTA; p++) {
     const int gp = a.t_off + q * PT + p;
-    kind[p] = (p < PT && gp < a.P) ? sh_tk[gp] : TK_NONE;
-    kidx[p] = (p < PT && gp < a.P) ? sh_ti[gp] : 0;
+    kind[p] = __builtin_amdgcn_readfirstlane((p < PT && gp < a.P) ? sh_tk[gp] : TK_NONE);
+    kidx[p] = __builtin_amdgcn_readfirstlane((p < PT && gp < a.P) ? sh_ti[gp] : 0);
   }
   double dwp[NC > 0 ? NC : 1];
 #pragma unroll
@@ -178,7 +186,7 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArg
   const int tib = team_index<TEAM>();
   double* keep = KEEP ? dyn_keep + (long)tib * keep_slot_doubles<NC, PT>() : nullptr;
   double* mir = MIR ? sh_mir[tib] : nullptr;
-  sim_team<NC, PT, TEAM, false, KEEP, MIR>(a, a.team0 + (long)blockIdx.x * TPB + tib, (long)a.V * a.O * a.n_pass, a.V, sh_prm,
+  sim_team<NC, PT, TEAM, false, KEEP, MIR>(a, a.team0 + (long)blockIdx.x * TPB + tib, pass_teams<TEAM>((long)a.V * a.O) * a.n_pass, a.V, sh_prm,
                          sh_state, sh_start, sh_tk, sh_ti, keep, mir);
 }
 
@@ -199,7 +207,7 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_list_kernel(S
   double* keep = KEEP ? dyn_keep + (long)tib * keep_slot_doubles<NC, PT>() : nullptr;
   double* mir = MIR ? sh_mir[tib] : nullptr;
   const int nv = *a.n_vlist;
-  const long total = (long)nv * a.O * a.n_pass;
+  const long total = pass_teams<TEAM>((long)nv * a.O) * a.n_pass;
   for (long base = (long)blockIdx.x * TPB; base < total; base += (long)gridDim.x * TPB) {
     sim_team<NC, PT, TEAM, true, KEEP, MIR>(a, base + tib, total, nv, sh_prm, sh_state, sh_start, sh_tk, sh_ti, keep, mir);
     __syncthreads();  // the next item overwrites the block's LDS
@@ -211,7 +219,7 @@ static int launch_nc_pt_k(const SimArgs& a, hipStream_t stream) {
   constexpr int NA = 3 * (1 + NC) + 2;
   constexpr int TEAM = NA <= 8 ? 8 : (NA <= 12 ? 12 : (NA <= 16 ? 16 : 32));  // one lane per propagator column
   constexpr int TPB = teams_per_block<TEAM>();
-  const long teams = (long)a.V * a.O * a.n_pass;
+  const long teams = pass_teams<TEAM>((long)a.V * a.O) * a.n_pass;
   const size_t dyn = KEEP ? (size_t)TPB * keep_slot_doubles<NC, PT>() * sizeof(double) : 0;
   if (dyn > 65536) {  // gfx950 gives a workgroup up to 160 KiB of LDS; dynamic LDS above 64 KiB is opt-in
     BMC_HIP_CHECK(hipFuncSetAttribute((const void*)sim_kernel<NC, PT, TEAM, KEEP, MIR>,
