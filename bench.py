@@ -28,11 +28,11 @@ sys.path[:0] = [os.path.join(ROOT, "bmc-jax_amd")]
 
 FREE = [("cest_k", 0), ("cest_f", 0), ("water_r1", None), ("water_r2", None), ("b0_inhom", None)]
 PEAK_FP64_TFLOPS = 78.6
-# HBM bytes per voxel of one launch of this workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 2.703e11 B
-# per 65,536-voxel step (its two dispatches: 58.9 GB fetched, 211.4 GB written), almost all
-# register-spill traffic (87 GB/s over the step, about 1 % of the HBM bandwidth).
-TRAFFIC_BYTES_PER_VOXEL = 2.703e11 / 65536
+# HBM bytes per voxel of one step of this workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 5.75e11 B
+# per 65,536-voxel step (its two dispatches: 201 GB fetched, 374 GB written), almost all
+# register-spill traffic (317 GB/s over the step, about 4 % of the HBM bandwidth).
+TRAFFIC_BYTES_PER_VOXEL = 5.75e11 / 65536
 
 
 def workload(n_vox, seed=2):
@@ -905,9 +905,19 @@ def main():
     pt = int(_native.lib().bmc_tangents_per_pass(len(slots)))
     per_vox = _flops.executed_flops(prm[sample].astype(np.float64), nc, ev_np, ops, rp, slots, pt=pt)
     flops_launch = float(per_vox.mean()) * V
+    # the same split by launch: passes of pt tangents, then the remainder alone (when P % pt != 0)
+    n_head = (len(slots) // pt) * pt if len(slots) % pt else len(slots)
+    by_launch = [float(_flops.executed_flops(prm[sample].astype(np.float64), nc, ev_np, ops, rp, slots[:n_head],
+                                             pt=pt).mean()) * V]
+    if n_head < len(slots):
+        by_launch.append(float(_flops.executed_flops(prm[sample].astype(np.float64), nc, ev_np, ops, rp,
+                                                     slots[n_head:], pt=len(slots) - n_head).mean()) * V)
     avg_launch_s = 1e-3 * float(np.mean(launch_ms))
     achieved = flops_launch / avg_launch_s / 1e12
-    # reference-equivalent flops (every event its own Pade-13-class exponential with 5 tangents)
+    # reference-equivalent flops: every event its own Pade exponential with 5 tangents, as the reference
+    # computes it (the "effective" rate beside the executed one)
+    ref_vox = _flops.reference_equivalent_flops(prm[sample[:4]].astype(np.float64), nc, ev_np, len(slots))
+    ref_step = float(ref_vox.mean()) * V
     stats = plan.stats()
 
     out = {
@@ -932,8 +942,11 @@ def main():
                      "and FP64 matrix peaks are both 78.6 TFLOP/s)", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES_PER_VOXEL * V,
                      "traffic_unit": "bytes per step (measured, scaled by voxels)",
-                     "launches_per_step": "2 (tangent passes 2 + 2, then the fifth tangent alone)",
-                     "flops_per_launch": flops_launch, "avg_launch_ms": 1e3 * avg_launch_s},
+                     "launches_per_step": "2 (tangent passes 2 + 2, then the fifth tangent alone); achieved = "
+                                          "executed flops of both / their time, HIP events on the launch stream",
+                     "flops_per_step": flops_launch, "flops_by_launch": by_launch, "avg_step_ms": 1e3 * avg_launch_s,
+                     "reference_equivalent_flops_per_spectrum": ref_step / V,
+                     "effective_tflops": ref_step / avg_launch_s / 1e12},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prm, m0, ev, nc, slots)

@@ -202,6 +202,31 @@ def reference_flops(n_aug, P, m_ref, s_ref):
     return 2 * n3 * (km + s_ref) + 8.0 / 3.0 * n3 + P * (4 * n3 * (km + s_ref) + 4 * n3) + 2 * n2 + 4 * P * n2
 
 
+def reference_equivalent_flops(prm, nc, events, P):
+    """The reference's work for the same rows, per voxel of prm [V, S]: every event with dt > 1e-9 its own
+    Pade exponential with P tangents and one apply (reference_flops), degree and squarings by the
+    reference float32 rule from the event's 1-norm. The "effective" figure beside the executed one."""
+    ev = {k: np.asarray(getattr(events, k) if not isinstance(events, dict) else events[k])
+          for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
+    n_aug = 3 * (1 + nc) + 2
+    total = np.zeros(prm.shape[0])
+    lo, hi, th = 0.4258730016922831, 1.880152677804762, 3.925724783138660
+    cost = {m: np.array([reference_flops(n_aug, P, m, s) for s in range(64)]) for m in (3, 5, 7)}
+    for o in range(ev["dt"].shape[0]):
+        dt = ev["dt"][o].astype(np.float64)
+        sel = np.flatnonzero(dt > 1e-9)
+        if sel.size == 0:
+            continue
+        ph = ev["rf_phase"][o, sel].astype(np.float64)
+        l1 = _gen_l1(prm, nc, dt[sel], ev["rf_amp"][o, sel].astype(np.float64), np.cos(ph), np.sin(ph),
+                     ev["rf_freq"][o, sel].astype(np.float64))
+        s = np.where(l1 > 0, np.maximum(0, np.floor(np.log2(np.maximum(l1, 1e-300) / th))), 0).astype(np.int64)
+        s = np.minimum(s, 63)
+        f = np.where(l1 < lo, cost[3][s], np.where(l1 < hi, cost[5][s], cost[7][s]))
+        total += f.sum(axis=1)
+    return total
+
+
 def reference_rule(l1):
     """Reference float32 Pade degree and squarings from the unscaled 1-norm (expm.py:73-77)."""
     s = 0 if l1 <= 0 else max(0, math.floor(math.log2(l1 / 3.925724783138660)))

@@ -180,3 +180,55 @@ def test_plan_marks_only_palindromes():
     ops, _, _ = _native.compile_program(skew["type"], skew["dt"], skew["rf_amp"], skew["rf_phase"],
                                         skew["rf_freq"], 3)
     assert np.any(ops[:, 0] == 1) and np.all(ops[:, 4] == 0)
+
+
+def sym_matrix(p, phi, flip_y=True):
+    """S of the palindrome factorisation at RF phase phi: R(phi) S0 R(-phi), S0 = diag over (x, y, z) of
+    water and each pool of (1, f_i) with the y block negated, and f_mt on MT's z (flip_y=False is a
+    plausible mistake that drops the sign)."""
+    f = np.asarray(p["cest_f"], float)
+    nt = 1 + f.size
+    N = 3 * nt + 1
+    sc = np.concatenate([[1.0], f])
+    s0 = np.concatenate([sc, (-1.0 if flip_y else 1.0) * sc, sc, [p["mt_f"]]])
+    R = rot_full(N, nt, phi)[:N, :N]
+    return R @ np.diag(s0) @ R.T
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.7, -2.3])
+def test_the_generator_is_symmetrisable_and_a_palindrome_is_its_transpose(phi):
+    """The identity behind plan level 3 (mirror_chain in csrc/bmc_device.h), on the oracle's generator and
+    exponential: S^-1 A S = A^T for every event, hence a chained run E_{2h-1} .. E_0 that reads the same
+    backwards equals Q^R Q with Q^R's linear part S Q_A^T S^-1 and its affine column S w, w the affine
+    column of exp(X'_0) .. exp(X'_{h-1}) for X' = [[A^T, S^-1 C], [0, 0]]. Without the y sign the
+    symmetrisation fails."""
+    p = four_pool_params(k_am=300.0, f_am=2e-3, b0i=0.05, rel_b1=1.1)
+    N = 3 * (1 + len(p["cest_k"])) + 1
+    S = sym_matrix(p, phi)
+    Si = np.linalg.inv(S)
+    amps = [3.0, 19.0, 47.5, 80.2]                 # the first half of a symmetric (Gaussian-like) pulse
+    for amp in amps:
+        A, _ = O.bmc_matrix(p, amp, phi, 420.0)
+        assert np.max(np.abs(Si @ A @ S - A.T)) < 1e-12 * np.max(np.abs(A))
+        Sb = sym_matrix(p, phi, flip_y=False)
+        assert np.max(np.abs(np.linalg.inv(Sb) @ A @ Sb - A.T)) > 1e-3 * np.max(np.abs(A))
+    dt = 1e-4
+    Es = [O.expm64(O.augmented(*O.bmc_matrix(p, a, phi, 420.0), dt)) for a in amps]
+    h = len(Es)
+    direct = np.eye(N + 1)
+    for E in Es + Es[::-1]:                        # E_0 .. E_{h-1}, then E_{h-1} .. E_0
+        direct = E @ direct
+    Q = np.eye(N + 1)
+    for E in Es:
+        Q = E @ Q
+    Wp = np.eye(N + 1)                             # exp(X'_0) .. exp(X'_{h-1}) from the transposed generators
+    for a in amps:
+        A, C = O.bmc_matrix(p, a, phi, 420.0)
+        Xp = np.zeros((N + 1, N + 1))
+        Xp[:N, :N], Xp[:N, N] = A.T, Si @ C
+        Wp = Wp @ O.expm64(Xp * dt)
+    QR = np.eye(N + 1)
+    QR[:N, :N] = S @ Q[:N, :N].T @ Si
+    QR[:N, N] = S @ Wp[:N, N]
+    assert np.max(np.abs(QR @ Q - direct)) < 1e-13
+    assert h == 4
