@@ -384,6 +384,9 @@ struct DxCo {
   int shape, ix;
   double q[7];
 };
+// Coefficients a kind can make nonzero in its shape (bit j for q[j]), forward and transposed
+// (dx_transpose) together: W_R1 q1, q3; W_R2 q0; C_K q0, q1, q2, q3, q5; C_F q0, q2, q6 and q1 transposed.
+constexpr int kQmWR1 = 0x0a, kQmWR2 = 0x01, kQmCK = 0x2f, kQmCF = 0x47;
 
 template <int NC>
 __device__ __forceinline__ DxCo dx_shape(int kind, int ix, const TanCo& t, const double* dwp) {
@@ -446,7 +449,16 @@ __device__ __forceinline__ DxCo dx_shape(int kind, int ix, const TanCo& t, const
 
 // y += dX v for a tangent in shape form (the same values as apply_dX with the kind it came from). With
 // SHAPE >= 0 the shape (and, for DXS_POOL, the pool IX) is fixed at compile time and d's are ignored.
-template <int NC, int SHAPE = -1, int IX = -1>
+// QM: the coefficients q[j] that can be nonzero (bit j), known at compile time when the tangent's kind
+// is; the FMAs of the others are not emitted (qm_of_kind).
+template <int QM, int J>
+__device__ __forceinline__ double qfma(const double* q, double a, double acc) {
+  if constexpr ((QM >> J) & 1)
+    return fma(q[J], a, acc);
+  else
+    return acc;
+}
+template <int NC, int SHAPE = -1, int IX = -1, int QM = 0x7f>
 __device__ __forceinline__ void apply_dx_shape(const DxCo& d, const double* v, double vaug, double* y) {
   constexpr int NT = 1 + NC;
   constexpr int xw = 0, yw = NT, zw = 2 * NT, zmt = 3 * NT;
@@ -454,22 +466,25 @@ __device__ __forceinline__ void apply_dx_shape(const DxCo& d, const double* v, d
   const int shape = SHAPE >= 0 ? SHAPE : d.shape;
   const int pix = IX >= 0 ? IX : d.ix;
   if (shape == DXS_WATER) {
-    y[xw] = fma(q[0], v[xw], y[xw]);
-    y[yw] = fma(q[0], v[yw], y[yw]);
-    y[zw] = fma(q[1], v[zw], fma(q[2], v[zmt], fma(q[3], vaug, y[zw])));
-    y[zmt] = fma(q[4], v[zw], fma(q[5], v[zmt], fma(q[6], vaug, y[zmt])));
+    y[xw] = qfma<QM, 0>(q, v[xw], y[xw]);
+    y[yw] = qfma<QM, 0>(q, v[yw], y[yw]);
+    y[zw] = qfma<QM, 1>(q, v[zw], qfma<QM, 2>(q, v[zmt], qfma<QM, 3>(q, vaug, y[zw])));
+    y[zmt] = qfma<QM, 4>(q, v[zw], qfma<QM, 5>(q, v[zmt], qfma<QM, 6>(q, vaug, y[zmt])));
   } else if (shape == DXS_POOL) {
 #pragma unroll
     for (int i = 0; i < NC; i++) {
       if (i != pix) continue;
       const int xc = 1 + i, yc = NT + 1 + i, zc = 2 * NT + 1 + i;
-      y[xw] = fma(q[0], v[xw], fma(q[1], v[xc], y[xw]));
-      y[yw] = fma(q[0], v[yw], fma(q[1], v[yc], y[yw]));
-      y[zw] = fma(q[0], v[zw], fma(q[1], v[zc], y[zw]));
+      y[xw] = qfma<QM, 0>(q, v[xw], qfma<QM, 1>(q, v[xc], y[xw]));
+      y[yw] = qfma<QM, 0>(q, v[yw], qfma<QM, 1>(q, v[yc], y[yw]));
+      y[zw] = qfma<QM, 0>(q, v[zw], qfma<QM, 1>(q, v[zc], y[zw]));
       const double vx = v[xc], vy = v[yc];
-      y[xc] = fma(q[2], v[xw], fma(q[3], vx, fma(q[4], vy, y[xc])));
-      y[yc] = fma(q[2], v[yw], fma(q[3], vy, fma(-q[4], vx, y[yc])));
-      y[zc] = fma(q[2], v[zw], fma(q[5], v[zc], fma(q[6], vaug, y[zc])));
+      y[xc] = qfma<QM, 2>(q, v[xw], qfma<QM, 3>(q, vx, qfma<QM, 4>(q, vy, y[xc])));
+      if constexpr ((QM >> 4) & 1)
+        y[yc] = fma(q[2], v[yw], fma(q[3], vy, fma(-q[4], vx, y[yc])));
+      else
+        y[yc] = qfma<QM, 2>(q, v[yw], qfma<QM, 3>(q, vy, y[yc]));
+      y[zc] = qfma<QM, 2>(q, v[zw], qfma<QM, 5>(q, v[zc], qfma<QM, 6>(q, vaug, y[zc])));
     }
   } else if (shape == DXS_SHIFT) {
     y[xw] = fma(q[0], v[yw], y[xw]);
@@ -750,7 +765,7 @@ __device__ void build_gen(const double* prm, const Ev& e, MtCache& mc, const int
 // is zero). acc / dacc may alias v0 / dv0 (in-place use): v0 is copied first.
 // The Horner recursion itself, over terms k = m..1, with the tangents' dX in shape form (SHAPE / IX as in
 // apply_dx_shape).
-template <int NC, int PT, int SHAPE, int IX>
+template <int NC, int PT, int SHAPE, int IX, int QM0 = 0x7f, int QM1 = 0x7f>
 __device__ __forceinline__ void taylor_terms(const Gen<NC>& G, const DxCo* dx, int m, const double* v, double vaug0,
                                              const double (*dv)[3 * (1 + NC) + 1], double* acc,
                                              double (*dacc)[3 * (1 + NC) + 1]) {
@@ -764,7 +779,10 @@ __device__ __forceinline__ void taylor_terms(const Gen<NC>& G, const DxCo* dx, i
     for (int p = 0; p < PT; p++) {
       double dy[N];
       apply_X<NC>(G, dacc[p], 0.0, dy);
-      apply_dx_shape<NC, SHAPE, IX>(dx[p], acc, vaug0, dy);  // dX at the current iterate
+      if (p == 0)  // dX at the current iterate
+        apply_dx_shape<NC, SHAPE, IX, QM0>(dx[p], acc, vaug0, dy);
+      else
+        apply_dx_shape<NC, SHAPE, IX, QM1>(dx[p], acc, vaug0, dy);
 #pragma unroll
       for (int i = 0; i < N; i++) dacc[p][i] = fma(dy[i], ik, dv[p][i]);
     }
@@ -822,6 +840,18 @@ __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const 
     for (int p = 1; p < PT; p++) same = same && dx[p].shape == dx[0].shape && dx[p].ix == dx[0].ix;
     if (same) {
       const int sh = dx[0].shape;
+      // the usual pairs of fitted values also at the kind level: only the coefficients the two kinds
+      // (forward or transposed) can make nonzero are applied
+      if constexpr (PT == 2) {
+        if (kind[0] == TK_W_R1 && kind[1] == TK_W_R2)
+          return taylor_terms<NC, PT, DXS_WATER, -1, kQmWR1, kQmWR2>(G, dx, m, v, vaug0, dv, acc, dacc);
+        if (kind[0] == TK_C_K && kind[1] == TK_C_F) {
+          if constexpr (NC > 0)
+            if (dx[0].ix == 0) return taylor_terms<NC, PT, DXS_POOL, 0, kQmCK, kQmCF>(G, dx, m, v, vaug0, dv, acc, dacc);
+          if constexpr (NC > 1)
+            if (dx[0].ix == 1) return taylor_terms<NC, PT, DXS_POOL, 1, kQmCK, kQmCF>(G, dx, m, v, vaug0, dv, acc, dacc);
+        }
+      }
       if (sh == DXS_WATER) return taylor_terms<NC, PT, DXS_WATER, -1>(G, dx, m, v, vaug0, dv, acc, dacc);
       if (sh == DXS_SHIFT) return taylor_terms<NC, PT, DXS_SHIFT, -1>(G, dx, m, v, vaug0, dv, acc, dacc);
       if (sh == DXS_B1) return taylor_terms<NC, PT, DXS_B1, -1>(G, dx, m, v, vaug0, dv, acc, dacc);
