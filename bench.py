@@ -790,6 +790,66 @@ def nn_record(args, log=None):
     return out
 
 
+def bench_squaring(args):
+    """The squarings' dense product on one GPU, two ways (bmc_jax._probe): the library's team column product
+    (mul_cols: DPP or ds_bpermute broadcasts of the left factor's columns) against one FP64 MFMA tile
+    (v_mfma_f64_16x16x4f64, factors padded to 16 x 16). reps products E <- P E on n matrices per launch."""
+    import torch
+    from bmc_jax import _probe
+    n, reps = 65536, 200
+    rng = np.random.default_rng(11)
+    recs = []
+    for nc in (1, 2, 3, 4):
+        na = 3 * (1 + nc) + 2
+        N = na - 1
+        # an orthogonal linear part keeps P^r E bounded over the repetitions
+        Q, _ = np.linalg.qr(rng.standard_normal((N, N)))
+        P = np.eye(na)
+        P[:N, :N] = Q
+        P[:N, N] = 0.01 * rng.standard_normal(N)
+        E = np.zeros((n, na, na))
+        E[:, :N, :] = rng.standard_normal((n, N, na))
+        E[:, N, N] = 1.0
+        Pd = torch.tensor(P, device="cuda")
+        outs = {}
+        for path, name in ((_probe.PATH_COLS, "team_columns"), (_probe.PATH_MFMA, "mfma_f64_16x16x4")):
+            if path == _probe.PATH_MFMA and na > 16:
+                continue
+            Ed = torch.tensor(E, device="cuda")
+            _probe.products(path, nc, Pd, Ed, 2)                     # warm-up
+            Ed.copy_(torch.tensor(E, device="cuda"))
+            stream = torch.cuda.current_stream()
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record(stream)
+            _probe.products(path, nc, Pd, Ed, reps)
+            t1.record(stream)
+            torch.cuda.synchronize()
+            ms = t0.elapsed_time(t1)
+            outs[name] = Ed.cpu().numpy()
+            rate = n * reps / (ms * 1e-3)
+            useful = 2.0 * N * N * na                                 # the augmented product's flops
+            team = 8 if na <= 8 else 12 if na <= 12 else 16 if na <= 16 else 32
+            recs.append({"n_cest": nc, "augmented": na, "path": name,
+                         "lanes": team if path == _probe.PATH_COLS else 64,
+                         "ms": ms, "products_per_s": rate, "useful_tflops": rate * useful / 1e12,
+                         "executed_tflops": rate * (useful if path == _probe.PATH_COLS else 2.0 * 16 ** 3) / 1e12})
+        if len(outs) == 2:
+            a, b = outs["team_columns"], outs["mfma_f64_16x16x4"]
+            recs[-1]["max_rel_diff_vs_team_columns"] = float(np.max(np.abs(a - b)) / np.max(np.abs(a)))
+    best = {}
+    for r in recs:
+        best.setdefault(r["n_cest"], {})[r["path"]] = r["products_per_s"]
+    out = {"metric": "augmented propagator products/s (the squarings' dense product), team columns vs FP64 MFMA",
+           "value": max(r["products_per_s"] for r in recs if r["n_cest"] == 2), "unit": "products/s",
+           "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded matrices)",
+           "config": {"workload": f"E <- P E, {reps} repetitions on {n} augmented matrices per launch, 1-4 CEST pools"},
+           "records": recs,
+           "mfma_over_team_columns": {nc: v.get("mfma_f64_16x16x4", float("nan")) / v["team_columns"]
+                                      for nc, v in best.items()}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -804,7 +864,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer (the reference point for scaling HBM counters)")
     ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven", "scanner", "joint", "tutorial08",
-                                           "tutorial09", "tutorial10", "tutorial11"), default="pulsed",
+                                           "tutorial09", "tutorial10", "tutorial11", "squaring"), default="pulsed",
                     help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
     ap.add_argument("--nn-train", type=int, default=2_000_000)
     ap.add_argument("--nn-test", type=int, default=262_144)
@@ -834,6 +894,10 @@ def main():
     if args.workload == "uneven":
         _native.torch_cuda()
         bench_uneven(args)
+        return
+    if args.workload == "squaring":
+        _native.torch_cuda()
+        bench_squaring(args)
         return
     if args.workload == "scanner":
         _native.torch_cuda()

@@ -16,6 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "bmc_jax", "_lib")
 LIB = os.path.join(OUT_DIR, "libbmc_hip.so")
+PROBE_LIB = os.path.join(OUT_DIR, "libbmc_probe.so")
 SOURCES = ["bmc_abi.hip", "bmc_sim.hip", "bmc_cwj.hip", "bmc_adj.hip", "bmc_uneven.hip", "bmc_lm.hip",
            "bmc_mlp.hip", "bmc_expm.hip"]
 ARCH = os.environ.get("BMC_OFFLOAD_ARCH", "gfx950")
@@ -78,7 +79,24 @@ def build(verbose: bool = False, jobs: int = 4, defines=(), incremental: bool = 
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_probe(incremental=incremental)
     return LIB
+
+
+def build_probe(incremental: bool = False) -> str:
+    """libbmc_probe.so: the product-path measurement probe (csrc/probe), apart from the ABI library."""
+    src = os.path.join(CSRC, "probe", "bmc_mfma_probe.hip")
+    if incremental and not _stale(PROBE_LIB, src):
+        return PROBE_LIB
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + inc + ["-shared", src, "-o", tmp]
+    out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if out.returncode != 0:
+        print(out.stdout.decode(errors="replace"), file=sys.stderr)
+        raise RuntimeError("hipcc failed for the probe library")
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
 
 
 if __name__ == "__main__":
