@@ -812,8 +812,9 @@ def bench_squaring(args):
         E[:, N, N] = 1.0
         Pd = torch.tensor(P, device="cuda")
         outs = {}
-        for path, name in ((_probe.PATH_COLS, "team_columns"), (_probe.PATH_MFMA, "mfma_f64_16x16x4")):
-            if path == _probe.PATH_MFMA and na > 16:
+        for path, name in ((_probe.PATH_COLS, "team_columns"), (_probe.PATH_MFMA, "mfma_f64_16x16x4"),
+                           (_probe.PATH_STAGED, "team_columns_lds_staged")):
+            if (path == _probe.PATH_MFMA and na > 16) or (path == _probe.PATH_STAGED and nc != 2):
                 continue
             Ed = torch.tensor(E, device="cuda")
             _probe.products(path, nc, Pd, Ed, 2)                     # warm-up
@@ -833,9 +834,10 @@ def bench_squaring(args):
                          "lanes": team if path == _probe.PATH_COLS else 64,
                          "ms": ms, "products_per_s": rate, "useful_tflops": rate * useful / 1e12,
                          "executed_tflops": rate * (useful if path == _probe.PATH_COLS else 2.0 * 16 ** 3) / 1e12})
-        if len(outs) == 2:
-            a, b = outs["team_columns"], outs["mfma_f64_16x16x4"]
-            recs[-1]["max_rel_diff_vs_team_columns"] = float(np.max(np.abs(a - b)) / np.max(np.abs(a)))
+        for r in recs:
+            if r["n_cest"] == nc and r["path"] != "team_columns" and r["path"] in outs:
+                a, b = outs["team_columns"], outs[r["path"]]
+                r["max_rel_diff_vs_team_columns"] = float(np.max(np.abs(a - b)) / np.max(np.abs(a)))
     best = {}
     for r in recs:
         best.setdefault(r["n_cest"], {})[r["path"]] = r["products_per_s"]

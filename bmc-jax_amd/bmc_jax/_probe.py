@@ -7,7 +7,7 @@ import os
 from . import _native
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libbmc_probe.so")
-PATH_COLS, PATH_MFMA = 0, 1
+PATH_COLS, PATH_MFMA, PATH_STAGED = 0, 1, 2
 _lib = None
 
 

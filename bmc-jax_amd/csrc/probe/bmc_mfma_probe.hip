@@ -40,6 +40,55 @@ __global__ void __launch_bounds__(256) probe_cols_kernel(const double* P, double
   }
 }
 
+// path 2 (12-lane teams): the same column product with the left factor's columns written to the team's
+// LDS stage once and read back as 16-byte broadcast loads instead of ds_bpermutes.
+template <int NC>
+__global__ void __launch_bounds__(256) probe_staged_kernel(const double* P, double* E, long n_mat, int reps) {
+  constexpr int N = 3 * (1 + NC) + 1, NA = N + 1, TEAM = 12;
+  static_assert(N % 2 == 0, "pairs of rows");
+  __shared__ __attribute__((aligned(16))) double stage[teams_per_block<TEAM>()][N * N];
+  const int lane = team_lane<TEAM>();
+  const int tib = team_index<TEAM>();
+  const long team = (long)blockIdx.x * teams_per_block<TEAM>() + tib;
+  const long t = team < n_mat ? team : 0;
+  const int j = lane < NA ? lane : NA - 1;
+  double a[N], b[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    a[i] = P[i * NA + j];
+    b[i] = E[t * NA * NA + i * NA + j];
+  }
+  volatile double* stw = stage[tib];
+  const double2* st2 = reinterpret_cast<const double2*>(stage[tib]);
+  for (int r = 0; r < reps; r++) {
+    team_lds_sync();
+    if (lane < N) {
+#pragma unroll
+      for (int i = 0; i < N; i++) stw[lane * N + i] = a[i];
+    }
+    team_lds_sync();
+    double o[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) o[i] = (lane == N) ? a[i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const double bk = b[k];
+#pragma unroll
+      for (int i2 = 0; i2 < N / 2; i2++) {
+        const double2 a2 = st2[(k * N) / 2 + i2];
+        o[2 * i2] = fma(a2.x, bk, o[2 * i2]);
+        o[2 * i2 + 1] = fma(a2.y, bk, o[2 * i2 + 1]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) b[i] = o[i];
+  }
+  if (team < n_mat && lane < NA) {
+#pragma unroll
+    for (int i = 0; i < N; i++) E[t * NA * NA + i * NA + j] = b[i];
+  }
+}
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <int NA>
@@ -91,7 +140,15 @@ template <int NC>
 static int launch(int path, const double* P, double* E, long n_mat, int reps, hipStream_t s) {
   constexpr int NA = 3 * (1 + NC) + 2;
   constexpr int TEAM = NA <= 8 ? 8 : (NA <= 12 ? 12 : (NA <= 16 ? 16 : 32));
-  if (path == 0) {
+  if (path == 2) {
+    if constexpr (TEAM == 12) {
+      constexpr int TPB = teams_per_block<TEAM>();
+      const long blocks = (n_mat + TPB - 1) / TPB;
+      hipLaunchKernelGGL((probe_staged_kernel<NC>), dim3((unsigned)blocks), dim3(256), 0, s, P, E, n_mat, reps);
+    } else {
+      return -2;  // the staged form is for 12-lane teams only
+    }
+  } else if (path == 0) {
     constexpr int TPB = teams_per_block<TEAM>();
     const long blocks = (n_mat + TPB - 1) / TPB;
     hipLaunchKernelGGL((probe_cols_kernel<NC, TEAM>), dim3((unsigned)blocks), dim3(256), 0, s, P, E, n_mat, reps);
@@ -107,11 +164,12 @@ static int launch(int path, const double* P, double* E, long n_mat, int reps, hi
 }  // namespace probe
 }  // namespace bmc
 
-// path 0 (team column product) or 1 (FP64 MFMA tile); nc = CEST pools (augmented size 3 (1 + nc) + 2).
+// path 0 (team column product), 1 (FP64 MFMA tile) or 2 (LDS-staged column product, 12-lane teams only);
+// nc = CEST pools (augmented size 3 (1 + nc) + 2).
 // P: NA x NA, E: n_mat x NA x NA, both row-major float64 device arrays. Returns 0, or a negative code for
 // an unsupported (path, nc), a size out of range or a launch error.
 extern "C" int bmc_probe_products(int path, int nc, const double* P, double* E, long n_mat, int reps, void* stream) {
-  if (n_mat < 1 || n_mat > (1L << 26) || reps < 0 || (path != 0 && path != 1)) return -1;
+  if (n_mat < 1 || n_mat > (1L << 26) || reps < 0 || path < 0 || path > 2) return -1;
   hipStream_t s = (hipStream_t)stream;
   switch (nc) {
     case 1: return bmc::probe::launch<1>(path, P, E, n_mat, reps, s);

@@ -15,13 +15,15 @@ def _aug(rng, n, na):
 
 
 @pytest.mark.parametrize("nc", [1, 2, 3, 4])
-@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("path", [0, 1, 2])
 def test_products_match_float64_numpy(nc, path):
     import torch
     from bmc_jax import _native, _probe
     _native.torch_cuda()
     if path == 1 and nc == 4:
         pytest.skip("17 x 17 does not fit one 16 x 16 MFMA tile (the probe returns an error for it)")
+    if path == 2 and nc != 2:
+        pytest.skip("the LDS-staged form is for 12-lane teams (two CEST pools) only")
     na = 3 * (1 + nc) + 2
     rng = np.random.default_rng(nc)
     P = _aug(rng, 1, na)[0]
