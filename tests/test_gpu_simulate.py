@@ -192,3 +192,32 @@ def test_configuration_2_full_train_golden(nat):
     for c in range(5):
         ref = g["jac"][..., c]
         assert np.max(np.abs(J[..., c] - ref)) < 1e-6 * np.abs(ref).max(), f"column {c}"
+
+
+def test_configuration_2_split_launch_golden(nat):
+    """The benchmark's launch shape: past 2^17 (voxel, offset) teams the five tangents run as passes of two
+    and then a launch of their own for the fifth (csrc/bmc_sim.hip launch_sim), with the palindromic pulses
+    factorised. The golden slice's 3 voxels tiled to 21,846 voxels x 6 offsets (131,076 teams), every copy
+    against the committed float64 oracle fixture, in float64 and with the benchmark's float32 I/O."""
+    import os
+    import torch
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg2_slice.npz"))
+    ev = O.pulsed_train(g["ppm"], b1_peak_uT=2.0, n_pulses=50, pulse_s=0.02, n_samples=200, gap_s=0.02,
+                        recovery_s=3.0, b0_T=3.0, dtype=np.float32)
+    plan = nat.Plan(ev["type"], ev["dt"], ev["rf_amp"], ev["rf_phase"], ev["rf_freq"])
+    reps = 7282
+    n_vox = 3 * reps
+    assert n_vox * len(g["ppm"]) >= 1 << 17
+    slots = [int(s) for s in g["slots"]]
+    for dtype, tz, tj in ((nat.BMC_F64, 1e-11, 1e-6), (nat.BMC_F32, 2e-6, 2e-4)):
+        tdt = torch.float64 if dtype == nat.BMC_F64 else torch.float32
+        prm = nat.device_array(np.tile(g["params"], (reps, 1)), tdt)
+        m0 = nat.device_array(np.tile(g["m0"], (reps, 1)), tdt)
+        m, J = nat.simulate(plan, prm, 2, dtype, m0, slots=slots, scales=[1.0] * 5, jc=6)
+        z = m.cpu().numpy().astype(np.float64)[..., 6].reshape(reps, 3, -1)
+        J = J.cpu().numpy().astype(np.float64).reshape(reps, 3, len(g["ppm"]), 5)
+        assert np.max(np.abs(z - g["z"][None])) < tz
+        for c in range(5):
+            ref = g["jac"][..., c]
+            assert np.max(np.abs(J[..., c] - ref[None])) < tj * np.abs(ref).max(), (dtype, c)
+        del m, J, prm, m0
