@@ -107,6 +107,14 @@ __device__ __forceinline__ double bcast(double x) {
   }
 }
 
+// Lanes of one team exchange values through LDS: no lane reads before every write issued earlier in
+// program order has landed (a team never spans two waves, whose LDS accesses complete in order).
+__device__ __forceinline__ void team_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // x plus the value of another lane of its DPP row, picked by the DPP control `CTRL`.
 template <int CTRL>
 __device__ __forceinline__ double dpp_add(double x) {
@@ -872,9 +880,14 @@ __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const 
 
 // out = A B for affine propagators in column layout (lane j holds column j, rows 0..N-1):
 // out[:, j] = sum_k A[:, k] B[k, j] + A[:, N] [j == N]; tangents by the product rule.
-template <int NC, int PT, int TEAM>
+// STAGED (12-lane teams, at most one tangent): 12-lane teams do not align with DPP rows, so their
+// broadcasts are ds_bpermutes, two per double; staged, the left factor's columns (then the tangent's) are
+// written to the team's LDS stage (N * N doubles) once and read back as 16-byte broadcast loads, 3.5x the
+// products per second in isolation. With two tangents the staged loop spilled: those kernels keep the
+// broadcasts. It raises the register pressure of a loop it is inlined into: it runs in staged_event.
+template <int NC, int PT, int TEAM, bool STAGED = false>
 __device__ __forceinline__ void mul_cols(const double* A, const double (*dA)[3 * (1 + NC) + 1], double* B,
-                                         double (*dB)[3 * (1 + NC) + 1], int j) {
+                                         double (*dB)[3 * (1 + NC) + 1], int j, double* stage = nullptr) {
   constexpr int N = 3 * (1 + NC) + 1;
   double o[N], od[PT > 0 ? PT : 1][N];
   const bool aff = (j == N);
@@ -884,21 +897,57 @@ __device__ __forceinline__ void mul_cols(const double* A, const double (*dA)[3 *
 #pragma unroll
     for (int p = 0; p < PT; p++) od[p][i] = aff ? dA[p][i] : 0.0;
   }
-  static_for<0, N>([&](auto kc) {
-    constexpr int k = decltype(kc)::value;
-    const double bk = B[k];
+  if constexpr (STAGED) {
+    static_assert(TEAM == 12 && N % 2 == 0 && PT <= 1, "the staged product: 12-lane teams, one tangent");
+    volatile double* stw = stage;  // volatile: a copy loop would become a memcpy from the registers
+    const double2* st2 = reinterpret_cast<const double2*>(stage);
+    static_for<0, 1 + PT>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;  // 0: A itself, then the tangent's dA
+      team_lds_sync();                        // the previous round's reads have landed
+      if (j < N) {
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-      const double a = bcast<TEAM, k>(A[i]);
-      o[i] = fma(a, bk, o[i]);
+        for (int i = 0; i < N; i++) stw[j * N + i] = r == 0 ? A[i] : dA[0][i];
+      }
+      team_lds_sync();
 #pragma unroll
-      for (int p = 0; p < PT; p++) od[p][i] = fma(a, dB[p][k], od[p][i]);
-    }
+      for (int k = 0; k < N; k++) {
+        const double bk = B[k];
 #pragma unroll
-    for (int p = 0; p < PT; p++)
+        for (int i2 = 0; i2 < N / 2; i2++) {
+          const double2 a2 = st2[(k * N) / 2 + i2];  // A[2 i2][k], A[2 i2 + 1][k], one load for the team
 #pragma unroll
-      for (int i = 0; i < N; i++) od[p][i] = fma(bcast<TEAM, k>(dA[p][i]), bk, od[p][i]);
-  });
+          for (int h = 0; h < 2; h++) {
+            const int i = 2 * i2 + h;
+            const double a = h == 0 ? a2.x : a2.y;
+            if constexpr (r == 0) {
+              o[i] = fma(a, bk, o[i]);
+              if constexpr (PT > 0) od[0][i] = fma(a, dB[0][k], od[0][i]);
+            } else {
+              od[0][i] = fma(a, bk, od[0][i]);
+            }
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);  // one staged column's loads in flight at a time
+      }
+    });
+    team_lds_sync();  // the stage is free again once every read of it has landed
+  } else {
+    static_for<0, N>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      const double bk = B[k];
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const double a = bcast<TEAM, k>(A[i]);
+        o[i] = fma(a, bk, o[i]);
+#pragma unroll
+        for (int p = 0; p < PT; p++) od[p][i] = fma(a, dB[p][k], od[p][i]);
+      }
+#pragma unroll
+      for (int p = 0; p < PT; p++)
+#pragma unroll
+        for (int i = 0; i < N; i++) od[p][i] = fma(bcast<TEAM, k>(dA[p][i]), bk, od[p][i]);
+    });
+  }
 #pragma unroll
   for (int i = 0; i < N; i++) {
     B[i] = o[i];
@@ -933,16 +982,16 @@ __device__ __forceinline__ void apply_cols(const double* A, const double (*dA)[3
 }
 
 // The propagator of one event in column layout: Taylor column + s squarings.
-template <int NC, int PT, int TEAM>
+template <int NC, int PT, int TEAM, bool STAGED = false>
 __device__ __forceinline__ void event_cols(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
                                            const double* dwp, int m, int s, bool is_nan, int j, double* R,
-                                           double (*dR)[3 * (1 + NC) + 1]) {
+                                           double (*dR)[3 * (1 + NC) + 1], double* stage = nullptr) {
   constexpr int N = 3 * (1 + NC) + 1;
 #pragma unroll
   for (int i = 0; i < N; i++) R[i] = (i == j) ? 1.0 : 0.0;
   taylor<NC, PT>(G, tc, kind, kidx, dwp, m, R, (j == N) ? 1.0 : 0.0, nullptr, R, dR);  // column j of T_m(X)
   // squarings: mul_cols finishes the product before it writes back, so R may be both factors
-  for (int q = 0; q < s; q++) mul_cols<NC, PT, TEAM>(R, dR, R, dR, j);
+  for (int q = 0; q < s; q++) mul_cols<NC, PT, TEAM, STAGED>(R, dR, R, dR, j, stage);
   if (is_nan) {
 #pragma unroll
     for (int i = 0; i < N; i++) {
@@ -953,6 +1002,74 @@ __device__ __forceinline__ void event_cols(const Gen<NC>& G, const TanCo* tc, co
   }
 }
 
+
+// An event that needs squarings, in 12-lane teams with an LDS stage and at most one tangent: its
+// propagator columns (Taylor series on the identity columns, then the squarings through the stage),
+// multiplied into the chained columns `Pi` when `chain` (else returned as they are). A call of its own
+// with its arguments and result passed by value, like mirror_chain, so that the staged products'
+// registers stay out of the event loop.
+template <int NC, int PT>
+struct ColsIn {  // the event's generator, its tangents and the chained columns
+  Gen<NC> G;
+  TanCo tc[PT > 0 ? PT : 1];
+  int kind[PT > 0 ? PT : 1], kidx[PT > 0 ? PT : 1];
+  double dwp[NC > 0 ? NC : 1];
+  double Pi[3 * (1 + NC) + 1];
+  double dPi[PT > 0 ? PT : 1][3 * (1 + NC) + 1];
+};
+template <int NC, int PT>
+struct ColsOut {
+  double R[3 * (1 + NC) + 1];
+  double dR[PT > 0 ? PT : 1][3 * (1 + NC) + 1];
+};
+
+template <int NC, int PT, int TEAM>
+__device__ __noinline__ ColsOut<NC, PT> staged_event(ColsIn<NC, PT> in, int m, int s, bool is_nan, int j, bool chain,
+                                                     double* stage_g) {
+  constexpr int N = 3 * (1 + NC) + 1;
+  constexpr int PTA = PT > 0 ? PT : 1;
+  double* stage = stage_g;
+  __builtin_assume(__builtin_amdgcn_is_shared((const __attribute__((address_space(0))) void*)stage));
+  // local copies: the by-value argument and the returned record live in scratch memory
+  const Gen<NC> G = in.G;
+  TanCo tc[PTA];
+  int kind[PTA], kidx[PTA];
+  double dwp[NC > 0 ? NC : 1];
+#pragma unroll
+  for (int p = 0; p < PTA; p++) {
+    tc[p] = in.tc[p];
+    kind[p] = in.kind[p];
+    kidx[p] = in.kidx[p];
+  }
+#pragma unroll
+  for (int i = 0; i < (NC > 0 ? NC : 1); i++) dwp[i] = in.dwp[i];
+  double R[N], dR[PTA][N];
+  event_cols<NC, PT, TEAM, true>(G, tc, kind, kidx, dwp, m, s, is_nan, j, R, dR, stage);
+  if (chain) {
+    double Pi[N], dPi[PTA][N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      Pi[i] = in.Pi[i];
+#pragma unroll
+      for (int p = 0; p < PTA; p++) dPi[p][i] = in.dPi[p][i];
+    }
+    mul_cols<NC, PT, TEAM, true>(R, dR, Pi, dPi, j, stage);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      R[i] = Pi[i];
+#pragma unroll
+      for (int p = 0; p < PTA; p++) dR[p][i] = dPi[p][i];
+    }
+  }
+  ColsOut<NC, PT> out;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    out.R[i] = R[i];
+#pragma unroll
+    for (int p = 0; p < PTA; p++) out.dR[p][i] = dR[p][i];
+  }
+  return out;
+}
 
 // M <- P M with P the kept propagator in LDS (lane j's column j at stride N + 1, tangent columns
 // after it), with tangents. Entries are read as they are used, so the kept block adds no registers
@@ -1082,13 +1199,6 @@ __device__ __forceinline__ void add_dsym(const double* prm, int kind, int ix, co
   }
 }
 
-// Lanes of one team exchange values through LDS: no lane reads before every write issued earlier in
-// program order has landed (a team never spans two waves, whose LDS accesses complete in order).
-__device__ __forceinline__ void team_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // The propagator of the palindromic run [e0, e0 + 2h) (with tangents) by its factorisation Q^R Q: lanes
 // 0..N chain the columns of Q over events e0 .. e0 + h - 1 as a chained block does, while lane N + 1
@@ -1258,7 +1368,9 @@ __device__ __noinline__ MirrorOut<NC, PT> mirror_chain(const double* prm_g, Mirr
 // a BLOCK and the kept-pulse code is not compiled in: it would cost the other plans registers.
 // With MIR, mir is the team's LDS stage of (N + 1) N doubles for the palindrome factorisation of chained
 // ops with a half (mirror_chain), in teams with a spare lane; without it those ops chain every event.
-template <int NC, int PT, int TEAM, bool KEEP = false, bool MIR = false>
+// With STAGE (12-lane teams, at most one tangent), mir is that stage also for the events that need
+// squarings, which then run in staged_event.
+template <int NC, int PT, int TEAM, bool KEEP = false, bool MIR = false, bool STAGE = false>
 __device__ __forceinline__ void run_row(const double* prm, const double* st0, double* st_park, const int* kind,
                                         const int* kidx, const double* dwp, const Ev* ev_row, const Op* ops, int op0,
                                         int op1, int reset_after_adc, double nan_l1, int lane,
@@ -1313,7 +1425,27 @@ __device__ __forceinline__ void run_row(const double* prm, const double* st0, do
             }
           } else {
             double R[N], dR[PTA][N];
-            event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
+            if constexpr (STAGE) {
+              ColsIn<NC, PT> in{};
+              in.G = G;
+#pragma unroll
+              for (int p = 0; p < PTA; p++) {
+                in.tc[p] = tc[p];
+                in.kind[p] = kind[p];
+                in.kidx[p] = kidx[p];
+              }
+#pragma unroll
+              for (int i = 0; i < (NC > 0 ? NC : 1); i++) in.dwp[i] = NC > 0 ? dwp[i] : 0.0;
+              const ColsOut<NC, PT> co = staged_event<NC, PT, TEAM>(in, m, s, isn, lane, false, mir);
+#pragma unroll
+              for (int i = 0; i < N; i++) {
+                R[i] = co.R[i];
+#pragma unroll
+                for (int p = 0; p < PTA; p++) dR[p][i] = co.dR[p][i];
+              }
+            } else {
+              event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
+            }
             apply_cols<NC, PT, TEAM>(R, dR, M, dM, lane);
           }
         }
@@ -1414,6 +1546,30 @@ __device__ __forceinline__ void run_row(const double* prm, const double* st0, do
 #pragma unroll
                 for (int p = 0; p < PTA; p++) dPi[p][i] = __builtin_nan("");
               }
+            }
+          } else if constexpr (STAGE) {
+            ColsIn<NC, PT> in{};
+            in.G = G;
+#pragma unroll
+            for (int p = 0; p < PTA; p++) {
+              in.tc[p] = tc[p];
+              in.kind[p] = kind[p];
+              in.kidx[p] = kidx[p];
+            }
+#pragma unroll
+            for (int i = 0; i < (NC > 0 ? NC : 1); i++) in.dwp[i] = NC > 0 ? dwp[i] : 0.0;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+              in.Pi[i] = Pi[i];
+#pragma unroll
+              for (int p = 0; p < PTA; p++) in.dPi[p][i] = dPi[p][i];
+            }
+            const ColsOut<NC, PT> co = staged_event<NC, PT, TEAM>(in, m, s, isn, lane, true, mir);
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+              Pi[i] = co.R[i];
+#pragma unroll
+              for (int p = 0; p < PTA; p++) dPi[p][i] = co.dR[p][i];
             }
           } else {
             double R[N], dR[PTA][N];

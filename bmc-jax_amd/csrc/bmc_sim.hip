@@ -47,11 +47,18 @@ __host__ __device__ constexpr long keep_slot_doubles() {
   return (long)(1 + (PT > 0 ? PT : 1)) * (3 * (1 + NC) + 1) * (3 * (1 + NC) + 2);
 }
 
-// Doubles of one team's LDS stage for the palindrome factorisation (mirror_chain): (N + 1) N in the
-// kernels that hold it (MIR), 1 (unused) in the others.
-template <int NC, bool MIR>
-__host__ __device__ constexpr int mirror_stage_doubles() {
-  return MIR ? (3 * (1 + NC) + 2) * (3 * (1 + NC) + 1) : 1;
+// Whether a kernel runs its squarings' products through the team's LDS stage (staged_event): 12-lane
+// teams with at most one tangent per pass.
+template <int TEAM, int PT>
+__host__ __device__ constexpr bool staged_products() {
+  return TEAM == 12 && PT <= 1;
+}
+// Doubles of one team's LDS stage: (N + 1) N in the kernels that factorise palindromes (MIR, for
+// mirror_chain) and in those with staged products (all plans then run the same products, bit for bit),
+// 1 (unused) in the others.
+template <int NC, int PT, int TEAM, bool MIR>
+__host__ __device__ constexpr int stage_doubles() {
+  return (MIR || staged_products<TEAM, PT>()) ? (3 * (1 + NC) + 2) * (3 * (1 + NC) + 1) : 1;
 }
 
 // One team's (voxel, offset row, pass) of a launch with `total` teams over n_vox voxels (voxel
@@ -146,7 +153,7 @@ __device__ __forceinline__ void sim_team(const SimArgs& a, long team, long total
   // state (identical on every lane of the team); the start state is reloaded from memory on a reset
   double M[N];
   double dM[PTA][N];
-  run_row<NC, PT, TEAM, KEEP, MIR>(prm, st0, sh_state[tib], kind, kidx, dwp, a.ev + (long)o * a.T, a.ops, a.row_ptr[o],
+  run_row<NC, PT, TEAM, KEEP, MIR, staged_products<TEAM, PT>()>(prm, st0, sh_state[tib], kind, kidx, dwp, a.ev + (long)o * a.T, a.ops, a.row_ptr[o],
                         a.row_ptr[o + 1], a.reset_after_adc, a.nan_l1, lane, M, dM,
                         (a.traj && !dead && q == 0) ? a.traj : nullptr, ((long)v * a.O + o) * a.T * N, a.dtype,
                         a.rot, keep, mir);
@@ -182,10 +189,10 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_kernel(SimArg
   __shared__ double sh_start[TPB][(1 + PTA) * N];  // the start state and its tangent seeds
   __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
   extern __shared__ double dyn_keep[];  // TPB keep slots when the plan holds KEEP ops
-  __shared__ double sh_mir[TPB][mirror_stage_doubles<NC, MIR>()];
+  __shared__ __attribute__((aligned(16))) double sh_mir[TPB][stage_doubles<NC, PT, TEAM, MIR>()];
   const int tib = team_index<TEAM>();
   double* keep = KEEP ? dyn_keep + (long)tib * keep_slot_doubles<NC, PT>() : nullptr;
-  double* mir = MIR ? sh_mir[tib] : nullptr;
+  double* mir = (MIR || staged_products<TEAM, PT>()) ? sh_mir[tib] : nullptr;
   sim_team<NC, PT, TEAM, false, KEEP, MIR>(a, a.team0 + (long)blockIdx.x * TPB + tib, pass_teams<TEAM>((long)a.V * a.O) * a.n_pass, a.V, sh_prm,
                          sh_state, sh_start, sh_tk, sh_ti, keep, mir);
 }
@@ -202,10 +209,10 @@ __global__ void __launch_bounds__(256, BMC_SIM_WAVES_PER_SIMD) sim_list_kernel(S
   __shared__ double sh_start[TPB][(1 + PTA) * N];
   __shared__ int sh_tk[kMaxTan], sh_ti[kMaxTan];
   extern __shared__ double dyn_keep[];
-  __shared__ double sh_mir[TPB][mirror_stage_doubles<NC, MIR>()];
+  __shared__ __attribute__((aligned(16))) double sh_mir[TPB][stage_doubles<NC, PT, TEAM, MIR>()];
   const int tib = team_index<TEAM>();
   double* keep = KEEP ? dyn_keep + (long)tib * keep_slot_doubles<NC, PT>() : nullptr;
-  double* mir = MIR ? sh_mir[tib] : nullptr;
+  double* mir = (MIR || staged_products<TEAM, PT>()) ? sh_mir[tib] : nullptr;
   const int nv = *a.n_vlist;
   const long total = pass_teams<TEAM>((long)nv * a.O) * a.n_pass;
   for (long base = (long)blockIdx.x * TPB; base < total; base += (long)gridDim.x * TPB) {
