@@ -221,3 +221,30 @@ def test_configuration_2_split_launch_golden(nat):
             ref = g["jac"][..., c]
             assert np.max(np.abs(J[..., c] - ref[None])) < tj * np.abs(ref).max(), (dtype, c)
         del m, J, prm, m0
+
+
+@pytest.mark.parametrize("n_tan", [0, 1])
+@pytest.mark.parametrize("level", [2, 3])
+def test_two_pool_kernels_with_staged_squarings(nat, n_tan, level):
+    """Kernels of 12-lane teams (two CEST pools) with at most one tangent per pass run the squarings'
+    products through the team's LDS stage (staged_event in csrc/bmc_device.h): the recovery delay as a
+    step of its own, the gaps inside the chained pulses, and CW rows (one event, about ten squarings) with
+    forward-mode tangents. Against the float64 oracle, both plan levels."""
+    import torch
+    recs = [four_pool_params(k_am=300.0, f_am=2e-3, b0i=0.05),
+            four_pool_params(k_am=2500.0, f_am=6e-3, t1w=2.1, b0i=-0.12, rel_b1=0.9)]
+    nc = 2
+    slots = [OC.slot_index("cest_f", 0, nc)] if n_tan else []
+    prm = nat.device_array(pack_many(recs), torch.float64)
+    m0 = nat.device_array(np.stack([O.initial_state(p) for p in recs]), torch.float64)
+    for ev, fwd in ((small_train(np.linspace(-5.0, 5.0, 7)), False),
+                    (O.cw_saturation(np.linspace(-6.0, 6.0, 9), b1_uT=1.5, tsat_s=2.0, b0_T=3.0), True)):
+        plan = nat.Plan(ev["type"], ev["dt"], ev["rf_amp"], ev["rf_phase"], ev["rf_freq"], compress=level)
+        m, J = nat.simulate(plan, prm, nc, nat.BMC_F64, m0, slots=slots, scales=[1.0] * len(slots),
+                            jc=2 * (1 + nc), forward_jacobian=fwd)
+        torch.cuda.synchronize()
+        mo, dmo = run_oracle(recs, ev, slots)
+        assert np.max(np.abs(m.cpu().numpy() - mo)) < 1e-11
+        if n_tan:
+            Jg, Jo = J.cpu().numpy()[..., 0], dmo[..., 0, 2 * (1 + nc)]
+            assert np.max(np.abs(Jg - Jo)) < 1e-7 * np.abs(Jo).max()
