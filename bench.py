@@ -778,7 +778,7 @@ def nn_record(args, log=None):
         "roofline": {"bound": "mfma", "kernel": "gemm_nt_kernel (bf16 MFMA 16x16x32, fp32 accumulate)",
                      "achieved": inf_tflops, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": inf_tflops / PEAK_BF16_TFLOPS,
-                     "measured_on": f"MLP inference over {Zt.shape[0]} spectra (batches of 4096), "
+                     "measured_on": f"MLP inference over {Zt.shape[0]} spectra (batches of {prop.mlp.max_batch}), "
                                     f"{1e3 * t_inf:.2f} ms including the input standardisation tiles",
                      "training_tflops": train_flops / max(r["train_seconds"], 1e-9) / 1e12},
     }

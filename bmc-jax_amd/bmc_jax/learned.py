@@ -119,6 +119,9 @@ def simulate_ladder(x, chunk=65536, device_out=False):
     return z if device_out else z.cpu().numpy()
 
 
+INFER_BATCH = 65536  # spectra per inference launch of a trained proposal
+
+
 class MLP:
     """Handle of a device MLP (bmc_mlp_*)."""
 
@@ -144,6 +147,14 @@ class MLP:
         _native.check(_native.lib().bmc_mlp_get_params(self._h, out.ctypes.data_as(ctypes.c_void_p)),
                       "bmc_mlp_get_params")
         return out
+
+    def copy_for_inference(self, max_batch):
+        """The same network (weights and input standardisation) in a handle sized for batches of max_batch:
+        a large spectrum set then runs in fewer, larger GEMM launches."""
+        other = MLP(self.dims, max_batch=max_batch)
+        other.set_params(self.get_params())
+        other.set_input_norm(*self.input_norm())
+        return other
 
     def set_params(self, w):
         w = np.ascontiguousarray(w, np.float32)
@@ -277,7 +288,10 @@ def train_proposal(n_train=2_000_000, epochs=30, batch=4096, lr=1e-3, noise=1e-3
     mlp.fit_input_norm(Z)
     hist = mlp.train(Z, T, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
     torch.cuda.synchronize()
-    return Proposal(mlp, hist, time.perf_counter() - t0)
+    t_train = time.perf_counter() - t0
+    del Z, T
+    # the trained network for inference over large sets: 65,536 spectra per launch instead of `batch`
+    return Proposal(mlp.copy_for_inference(INFER_BATCH), hist, t_train)
 
 
 def propose(prop: Proposal, Z):

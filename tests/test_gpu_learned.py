@@ -107,3 +107,34 @@ def test_the_trained_proposal_beats_the_plain_start(torch_mod):
         zf = learned.simulate_ladder(X)
         rms[name] = np.median(np.sqrt(np.mean((zf - Z) ** 2, axis=1)))
     assert rms["nn"] < rms["plain"], rms
+
+
+def test_an_inference_copy_gives_the_same_outputs(torch_mod):
+    """copy_for_inference (weights and input standardisation in a handle sized for larger batches, the
+    trained proposal's inference handle) against the training handle, which runs the same spectra in
+    chunks of its own batch size, and both against the oracle."""
+    torch = torch_mod
+    from bmc_jax.learned import DIMS, MLP
+    mlp = MLP(dims=DIMS, max_batch=512, seed=7)
+    rng = np.random.default_rng(1)
+    # Z-spectrum-like inputs (as in the network's own test): a mean near 1 per feature, a few percent spread
+    X = torch.as_tensor((0.9 - 0.3 * np.sin(np.linspace(0, 3, DIMS[0])) + 0.03 * rng.normal(size=(3000, DIMS[0])))
+                        .astype(np.float32), device="cuda")
+    mlp.fit_input_norm(X)
+    big = mlp.copy_for_inference(4096)
+    assert big.max_batch == 4096 and big.n_params == mlp.n_params
+    np.testing.assert_array_equal(big.get_params(), mlp.get_params())
+    for a, b in zip(big.input_norm(), mlp.input_norm()):
+        np.testing.assert_array_equal(a, b)
+    # the hidden activations are rounded to BF16 between layers, so a different GEMM split (the batch
+    # sizes differ) may round a sum to the neighbouring BF16 value: most outputs agree bit for bit, and
+    # the copy is no further from the float64 oracle (same rounding points) than the original
+    y_small = mlp.forward(X).cpu().numpy()
+    y_big = big.forward(X).cpu().numpy()
+    assert np.mean(y_big == y_small) > 0.5
+    shift, scale = mlp.input_norm()
+    Yo, _, _ = MO.forward(mlp.get_params().astype(np.float64), DIMS, X.cpu().numpy(), shift, scale)
+    e_small, e_big = np.abs(y_small - Yo), np.abs(y_big - Yo)
+    scale_y = max(1.0, np.abs(Yo).max())
+    assert e_big.max() <= max(1.5 * e_small.max(), 2e-3 * scale_y)
+    assert np.median(e_big) < 1e-4 * scale_y
