@@ -1600,7 +1600,19 @@ __device__ __forceinline__ void run_row(const double* prm, const double* st0, do
 #pragma unroll
         for (int p = 0; p < PTA; p++) dM[p][i] = st[(1 + p) * N + i];
       }
-      for (int c = 0; c < op.count; c++) apply_cols<NC, PT, TEAM>(Pi, dPi, M, dM, lane);
+      if (op.count < 4) {
+        for (int c = 0; c < op.count; c++) apply_cols<NC, PT, TEAM>(Pi, dPi, M, dM, lane);
+      } else {
+        // M <- Pi^count M by binary powering: about log2(count) squarings and popcount(count)
+        // applications instead of count applications, each of which costs (1 + PT) N team sums
+        // (the powers of Pi commute, so the order of the applications does not matter)
+        for (int c = op.count;;) {
+          if (c & 1) apply_cols<NC, PT, TEAM>(Pi, dPi, M, dM, lane);
+          c >>= 1;
+          if (c == 0) break;
+          mul_cols<NC, PT, TEAM>(Pi, dPi, Pi, dPi, lane);  // Pi <- Pi^2 (the product finishes before it writes)
+        }
+      }
     }
   }
 
