@@ -29,10 +29,10 @@ sys.path[:0] = [os.path.join(ROOT, "bmc-jax_amd")]
 FREE = [("cest_k", 0), ("cest_f", 0), ("water_r1", None), ("water_r2", None), ("b0_inhom", None)]
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per voxel of one step of this workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 9.97e11 B
-# per 65,536-voxel step (its two dispatches: 295 GB fetched, 702 GB written), almost all
-# register-spill and call-frame traffic (560 GB/s over the step, about 7 % of the HBM bandwidth).
-TRAFFIC_BYTES_PER_VOXEL = 9.97e11 / 65536
+# runs scaled by a 1 GiB copy kernel's readings (profiles/sim_kernel_pulsed_train.md): 1.060e12 B
+# per 65,536-voxel step (its two dispatches: 307 GB fetched, 753 GB written), almost all
+# register-spill and call-frame traffic (627 GB/s over the step, about 8 % of the HBM bandwidth).
+TRAFFIC_BYTES_PER_VOXEL = 1.060e12 / 65536
 
 
 def workload(n_vox, seed=2):

@@ -7,6 +7,11 @@ count s (the same 1-norm and theta_m table) and counts the useful float64 operat
   Taylor term bookkeeping   2 N per vector (scale by 1/k, accumulate)
   dense product (column)    2 N^2 (1 + 2 PT) per column, N + 1 useful columns (squarings, and chaining
                             an event that needed squarings; other chained events need none)
+  block product (column)    2 (NXY^2 + NZ^2) (1 + 2 PT) per column for an event without RF (rf_amp 0):
+                            its propagator is block-diagonal in the transverse rows (NXY = 2 (1 + NC))
+                            and the z rows (NZ = N - NXY), and the kernel multiplies the blocks apart
+  chained block applied     count times for count < 4; from 4 on by binary powering: popcount(count)
+                            applications and bit_length(count) - 1 squarings of the chained propagator
   apply to the state        (N + 1)(N + 3 PT N) + N (1 + PT) N adds of the team reduction
   palindrome factorisation  a chained run opening with 2h mirrored events (plan level 3; teams with a
                             spare lane, plans without kept pulses) chains h events on N + 2 columns
@@ -140,6 +145,7 @@ def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1, mirror=True):
           for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
     N = 3 * (1 + nc) + 1
     NA = N + 1
+    NXY = 2 * (1 + nc)
     FX = 29 * nc + 27
     P = len(slots)
     passes = [slots[i:i + pt] for i in range(0, P, pt)] if P else [[]]
@@ -168,20 +174,22 @@ def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1, mirror=True):
                          np.cos(ev["rf_phase"][o, sel].astype(np.float64)),
                          np.sin(ev["rf_phase"][o, sel].astype(np.float64)), ev["rf_freq"][o, sel].astype(np.float64))
             m, s = degree_and_squarings(l1)
+            rf0 = ev["rf_amp"][o, sel] == 0          # events without RF: block-diagonal products
             for ps in passes:
                 T = len(ps)
                 fdx = sum(dx_flops(sl, nc) for sl in ps)
                 term = FX + T * FX + fdx + 2 * N * (1 + T)
                 mul = NA * 2 * N * N * (1 + 2 * T)
+                mulx = np.where(rf0, NA * 2 * (NXY * NXY + (N - NXY) ** 2) * (1 + 2 * T), mul)
                 app = NA * (N + 3 * T * N) + N * (1 + T) * (NA - 1)
                 if kind == 0:
                     vec = (s == 0)
-                    f = np.where(vec, m * term, NA * m * term + s * mul + app)
+                    f = np.where(vec, m * term, NA * m * term + s * mulx + app)
                     total += f.sum(axis=1)
                 else:
                     # chained: an event without squarings runs its series on the chained columns
                     # directly; one with squarings forms its propagator and multiplies it in
-                    f = np.where(s == 0, NA * m * term, NA * m * term + s * mul + mul)
+                    f = np.where(s == 0, NA * m * term, NA * m * term + s * mulx + mulx)
                     plain = f.sum(axis=1)
                     if can_mirror and half > 0:
                         # events [e0, e0 + 2h) with dt > 1e-9 are the first live ones of sel
@@ -191,8 +199,16 @@ def executed_flops(prm, nc, events, ops, row_ptr, slots, pt=1, mirror=True):
                         mir = (NA + 1) * (m[:, :nh] * term).sum(axis=1) + NA * 2 * N * N * (1 + 2 * T) \
                             + f[:, n2:].sum(axis=1)
                         plain = np.where(ok, mir, plain)
-                    total += plain + cnt * app
+                    total += plain + power_flops(cnt, app, mul)
     return total
+
+
+def power_flops(count, app, mul):
+    """Flops of applying a chained propagator count times to the state (csrc/bmc_device.h run_row):
+    count applications below 4, else binary powering (popcount applications, bit_length - 1 squarings)."""
+    if count < 4:
+        return count * app
+    return bin(count).count("1") * app + (count.bit_length() - 1) * mul
 
 
 def reference_flops(n_aug, P, m_ref, s_ref):
@@ -238,7 +254,7 @@ def adjoint_flops(prm, nc, events, slots, n_q=2):
     """Useful float64 flops of one bmc_simulate_adjoint call, per voxel of prm [V, S] (summed over rows).
 
     Per live event: the forward step twice (checkpoint sweep and segment recompute; all P tangents in
-    one pass) and, for each of the n_q RF directions, the second-order columns (Taylor terms of the
+    one pass; an event without RF squares its two diagonal blocks apart) and, for each of the n_q RF directions, the second-order columns (Taylor terms of the
     state, P tangents, the RF direction and P mixed terms; squarings by the product rule) and the
     adjoint dot products. Idle lanes of a team are not counted.
     """
@@ -246,6 +262,7 @@ def adjoint_flops(prm, nc, events, slots, n_q=2):
           for k in ("type", "dt", "rf_amp", "rf_phase", "rf_freq")}
     N = 3 * (1 + nc) + 1
     NA = N + 1
+    NXY = 2 * (1 + nc)
     FX = 29 * nc + 27
     P = len(slots)
     fdx = sum(dx_flops(sl, nc) for sl in slots)
@@ -266,7 +283,8 @@ def adjoint_flops(prm, nc, events, slots, n_q=2):
         l1 = _gen_l1(prm, nc, dt[sel], ev["rf_amp"][o, sel].astype(np.float64), np.cos(ph), np.sin(ph),
                      ev["rf_freq"][o, sel].astype(np.float64))
         m, s = degree_and_squarings(l1)
-        fwd = np.where(s == 0, m * term, NA * m * term + s * mul + app)
+        mulx = np.where(ev["rf_amp"][o, sel] == 0, NA * 2 * (NXY * NXY + (N - NXY) ** 2) * (1 + 2 * P), mul)
+        fwd = np.where(s == 0, m * term, NA * m * term + s * mulx + app)
         adj = 0.0
         for dq in dxq:
             term2 = FX * (2 + 2 * P) + fdx + dq + P * dq + fdx + 2 * N * (2 + 2 * P)

@@ -885,10 +885,14 @@ __device__ __forceinline__ void taylor(const Gen<NC>& G, const TanCo* tc, const 
 // written to the team's LDS stage (N * N doubles) once and read back as 16-byte broadcast loads, 3.5x the
 // products per second in isolation. With two tangents the staged loop spilled: those kernels keep the
 // broadcasts. It raises the register pressure of a loop it is inlined into: it runs in staged_event.
-template <int NC, int PT, int TEAM, bool STAGED = false>
+// BD: A and its tangents are block-diagonal, transverse rows [0, 2 NT) apart from the z rows [2 NT, N)
+// (an event without RF, rf_free): column k of A is broadcast and multiplied over its own block's rows
+// only. The terms left out are exact zeros and the others accumulate in the same order, so the result
+// is the same as the dense product's.
+template <int NC, int PT, int TEAM, bool STAGED = false, bool BD = false>
 __device__ __forceinline__ void mul_cols(const double* A, const double (*dA)[3 * (1 + NC) + 1], double* B,
                                          double (*dB)[3 * (1 + NC) + 1], int j, double* stage = nullptr) {
-  constexpr int N = 3 * (1 + NC) + 1;
+  constexpr int N = 3 * (1 + NC) + 1, NXY = 2 * (1 + NC);
   double o[N], od[PT > 0 ? PT : 1][N];
   const bool aff = (j == N);
 #pragma unroll
@@ -912,8 +916,9 @@ __device__ __forceinline__ void mul_cols(const double* A, const double (*dA)[3 *
 #pragma unroll
       for (int k = 0; k < N; k++) {
         const double bk = B[k];
+        const int i0 = (BD && k >= NXY) ? NXY : 0, i1 = (BD && k < NXY) ? NXY : N;  // NXY is even
 #pragma unroll
-        for (int i2 = 0; i2 < N / 2; i2++) {
+        for (int i2 = i0 / 2; i2 < i1 / 2; i2++) {
           const double2 a2 = st2[(k * N) / 2 + i2];  // A[2 i2][k], A[2 i2 + 1][k], one load for the team
 #pragma unroll
           for (int h = 0; h < 2; h++) {
@@ -934,9 +939,10 @@ __device__ __forceinline__ void mul_cols(const double* A, const double (*dA)[3 *
   } else {
     static_for<0, N>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
+      constexpr int i0 = (BD && k >= NXY) ? NXY : 0, i1 = (BD && k < NXY) ? NXY : N;
       const double bk = B[k];
 #pragma unroll
-      for (int i = 0; i < N; i++) {
+      for (int i = i0; i < i1; i++) {
         const double a = bcast<TEAM, k>(A[i]);
         o[i] = fma(a, bk, o[i]);
 #pragma unroll
@@ -945,7 +951,7 @@ __device__ __forceinline__ void mul_cols(const double* A, const double (*dA)[3 *
 #pragma unroll
       for (int p = 0; p < PT; p++)
 #pragma unroll
-        for (int i = 0; i < N; i++) od[p][i] = fma(bcast<TEAM, k>(dA[p][i]), bk, od[p][i]);
+        for (int i = i0; i < i1; i++) od[p][i] = fma(bcast<TEAM, k>(dA[p][i]), bk, od[p][i]);
     });
   }
 #pragma unroll
@@ -981,17 +987,36 @@ __device__ __forceinline__ void apply_cols(const double* A, const double (*dA)[3
   for (int i = 0; i < N; i++) M[i] = team_sum<TEAM>(A[i] * mj);
 }
 
-// The propagator of one event in column layout: Taylor column + s squarings.
+// An event without RF: its generator and every tangent's derivative of it have no term between the
+// transverse rows (x, y of every pool) and the z rows (z of every pool, the MT pool, the affine
+// column), so its propagator is block-diagonal in them. Only the RF vector (w1x, w1y) and the B1 tangent,
+// whose coefficients are derivatives of it, couple the two.
+template <int NC, int PT>
+__device__ __forceinline__ bool rf_free(const Gen<NC>& G, const TanCo* tc, const int* kind) {
+  bool f = G.w1x == 0.0 && G.w1y == 0.0;
+#pragma unroll
+  for (int p = 0; p < PT; p++) f = f && (kind[p] != TK_RELB1 || (tc[p].c0 == 0.0 && tc[p].c1 == 0.0));
+  return f;
+}
+
+// The propagator of one event in column layout: Taylor column + s squarings. Returns rf_free: the
+// propagator is then block-diagonal (mul_cols' BD form applies to it).
 template <int NC, int PT, int TEAM, bool STAGED = false>
-__device__ __forceinline__ void event_cols(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
+__device__ __forceinline__ bool event_cols(const Gen<NC>& G, const TanCo* tc, const int* kind, const int* kidx,
                                            const double* dwp, int m, int s, bool is_nan, int j, double* R,
                                            double (*dR)[3 * (1 + NC) + 1], double* stage = nullptr) {
   constexpr int N = 3 * (1 + NC) + 1;
 #pragma unroll
   for (int i = 0; i < N; i++) R[i] = (i == j) ? 1.0 : 0.0;
   taylor<NC, PT>(G, tc, kind, kidx, dwp, m, R, (j == N) ? 1.0 : 0.0, nullptr, R, dR);  // column j of T_m(X)
-  // squarings: mul_cols finishes the product before it writes back, so R may be both factors
-  for (int q = 0; q < s; q++) mul_cols<NC, PT, TEAM, STAGED>(R, dR, R, dR, j, stage);
+  // squarings: mul_cols finishes the product before it writes back, so R may be both factors; an event
+  // without RF (the delays) squares its two blocks apart
+  const bool bd = rf_free<NC, PT>(G, tc, kind);
+  if (bd) {
+    for (int q = 0; q < s; q++) mul_cols<NC, PT, TEAM, STAGED, true>(R, dR, R, dR, j, stage);
+  } else {
+    for (int q = 0; q < s; q++) mul_cols<NC, PT, TEAM, STAGED>(R, dR, R, dR, j, stage);
+  }
   if (is_nan) {
 #pragma unroll
     for (int i = 0; i < N; i++) {
@@ -1000,6 +1025,7 @@ __device__ __forceinline__ void event_cols(const Gen<NC>& G, const TanCo* tc, co
       for (int p = 0; p < PT; p++) dR[p][i] = __builtin_nan("");
     }
   }
+  return bd;
 }
 
 
@@ -1044,7 +1070,7 @@ __device__ __noinline__ ColsOut<NC, PT> staged_event(ColsIn<NC, PT> in, int m, i
 #pragma unroll
   for (int i = 0; i < (NC > 0 ? NC : 1); i++) dwp[i] = in.dwp[i];
   double R[N], dR[PTA][N];
-  event_cols<NC, PT, TEAM, true>(G, tc, kind, kidx, dwp, m, s, is_nan, j, R, dR, stage);
+  const bool bd = event_cols<NC, PT, TEAM, true>(G, tc, kind, kidx, dwp, m, s, is_nan, j, R, dR, stage);
   if (chain) {
     double Pi[N], dPi[PTA][N];
 #pragma unroll
@@ -1053,7 +1079,10 @@ __device__ __noinline__ ColsOut<NC, PT> staged_event(ColsIn<NC, PT> in, int m, i
 #pragma unroll
       for (int p = 0; p < PTA; p++) dPi[p][i] = in.dPi[p][i];
     }
-    mul_cols<NC, PT, TEAM, true>(R, dR, Pi, dPi, j, stage);
+    if (bd)
+      mul_cols<NC, PT, TEAM, true, true>(R, dR, Pi, dPi, j, stage);
+    else
+      mul_cols<NC, PT, TEAM, true>(R, dR, Pi, dPi, j, stage);
 #pragma unroll
     for (int i = 0; i < N; i++) {
       R[i] = Pi[i];
@@ -1573,8 +1602,10 @@ __device__ __forceinline__ void run_row(const double* prm, const double* st0, do
             }
           } else {
             double R[N], dR[PTA][N];
-            event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR);
-            mul_cols<NC, PT, TEAM>(R, dR, Pi, dPi, lane);
+            if (event_cols<NC, PT, TEAM>(G, tc, kind, kidx, dwp, m, s, isn, lane, R, dR))
+              mul_cols<NC, PT, TEAM, false, true>(R, dR, Pi, dPi, lane);  // a delay: block-diagonal R
+            else
+              mul_cols<NC, PT, TEAM>(R, dR, Pi, dPi, lane);
           }
         }
         if (e.type == BMC_EV_GRADIENT || e.type == BMC_EV_SPOILER) {
