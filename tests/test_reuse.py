@@ -1,6 +1,7 @@
-"""Pulses reused across a row with a shifted RF phase (plan compression level 2), and the event-cache
-format. CPU only: the rotation identity on the oracle, the host compilation (bmc_plan_compile), the
-cache reader and writer.
+"""Pulses reused across a row with a shifted RF phase (plan compression level 2), the event-cache
+format, and the block structure of events without RF that the kernels' delay products rely on. CPU only:
+the rotation identity on the oracle, the host compilation (bmc_plan_compile), the cache reader and
+writer, the generator's blocks.
 
 A converted scanner sequence gives every pulse of a train the phase accumulated by the pulses before it
 (the reference's sequences/pulseq.py:56-66, 118-130), so no two pulses are equal and a plan can only
@@ -232,3 +233,46 @@ def test_the_generator_is_symmetrisable_and_a_palindrome_is_its_transpose(phi):
     QR[:N, N] = S @ Wp[:N, N]
     assert np.max(np.abs(QR @ Q - direct)) < 1e-13
     assert h == 4
+
+
+FIELDS = [("water_r1", None), ("water_r2", None), ("water_f", None), ("mt_k", None), ("mt_f", None),
+          ("mt_dw", None), ("mt_r1", None), ("mt_r2", None), ("b0", None), ("gamma", None), ("b0_inhom", None),
+          ("rel_b1", None), ("cest_k", 0), ("cest_f", 0), ("cest_dw", 0), ("cest_r1", 0), ("cest_r2", 0),
+          ("cest_k", 1), ("cest_f", 1), ("cest_dw", 1), ("cest_r1", 1), ("cest_r2", 1)]
+
+
+def _off_blocks(X, nt):
+    """The two off-diagonal blocks of an augmented matrix split into the transverse rows [0, 2 nt) and
+    the z rows with MT's z and the homogeneous row [2 nt, N + 1)."""
+    xy = slice(0, 2 * nt)
+    z = slice(2 * nt, X.shape[0])
+    return np.concatenate([X[xy, z].ravel(), X[z, xy].ravel()])
+
+
+@pytest.mark.parametrize("lineshape", ["superlorentzian", "lorentzian"])
+def test_an_event_without_rf_is_block_diagonal(lineshape):
+    """The kernels square a delay's propagator and multiply it into a chained one block by block
+    (rf_free / mul_cols' BD form in csrc/bmc_device.h). That rests on this: without RF the generator
+    (matrix.py:83-295, restated as bmc_matrix) and its derivative with respect to every packed field have
+    no entry between the transverse rows and the z rows, so the exponential and its Frechet derivatives
+    have none either. With RF both the generator and the B1 derivative couple the blocks (the plausible
+    mistake of taking an RF event for a delay)."""
+    p = four_pool_params(k_am=1200.0, f_am=5e-3, b0i=-0.2, rel_b1=0.9, lineshape=lineshape)
+    nt = 1 + len(p["cest_k"])
+    for freq, dt in ((-300.0, 0.02), (616.0, 3.0), (0.0, 1e-3)):
+        X = O.augmented(*O.bmc_matrix(p, 0.0, 0.0, freq), dt)
+        assert np.all(_off_blocks(X, nt) == 0.0)
+        assert np.abs(X).sum() > 0
+        E = O.expm64(X)
+        assert np.max(np.abs(_off_blocks(E, nt))) < 1e-15
+        for field, idx in FIELDS:
+            v = p[field] if idx is None else p[field][idx]
+            h = 1e-6 * max(abs(v), 1e-3)
+            Xp = O.augmented(*O.bmc_matrix(O.write_params(p, [v + h], [(field, idx)]), 0.0, 0.0, freq), dt)
+            Xm = O.augmented(*O.bmc_matrix(O.write_params(p, [v - h], [(field, idx)]), 0.0, 0.0, freq), dt)
+            assert np.all(_off_blocks(Xp, nt) == 0.0) and np.all(_off_blocks(Xm, nt) == 0.0), field
+        # with RF the blocks are coupled, and so is the rel_b1 derivative
+        Xr = O.augmented(*O.bmc_matrix(p, 40.0, 0.3, freq), dt)
+        assert np.max(np.abs(_off_blocks(Xr, nt))) > 1e-3
+        Xr2 = O.augmented(*O.bmc_matrix(O.write_params(p, [1.0], [("rel_b1", None)]), 40.0, 0.3, freq), dt)
+        assert np.max(np.abs(_off_blocks(Xr2 - Xr, nt))) > 1e-4
