@@ -35,11 +35,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
-  // round to nearest even
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16) | ((u & 0xffffu) ? 0x40 : 0);  // inf / nan
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  // round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 
 inline long pad8(long n) { return (n + 7) & ~7L; }
@@ -74,17 +71,40 @@ struct GemmArgs {
   Epi e;
 };
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// erfc(z) by the Chebyshev fit of Numerical Recipes (Press et al., routine erfcc): with t = 1 / (1 + |z| / 2),
+// erfc(|z|) = t exp(-z^2 + P(t)), P of degree 9, fractional error below 1.2e-7 in exact arithmetic (a few
+// float32 ulp more here), and erfc(-z) = 2 - erfc(z). Branch-free: one reciprocal and one exponential
+// per value, where a two-range erf diverges within a wave.
+__device__ __forceinline__ float erfc_cheb(float z) {
+  const float a = fabsf(z);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, a, 1.0f));  // v_rcp_f32, 1 ulp
+  float p = 0.17087277f;
+  p = fmaf(t, p, -0.82215223f);
+  p = fmaf(t, p, 1.48851587f);
+  p = fmaf(t, p, -1.13520398f);
+  p = fmaf(t, p, 0.27886807f);
+  p = fmaf(t, p, -0.18628806f);
+  p = fmaf(t, p, 0.09678418f);
+  p = fmaf(t, p, 0.37409196f);
+  p = fmaf(t, p, 1.00002368f);
+  p = fmaf(t, p, -1.26551223f);
+  const float e = t * __expf(fmaf(-a, a, p));
+  return z >= 0.f ? e : 2.0f - e;
+}
+// GELU x Phi(x) = x erfc(-x / sqrt 2) / 2 (no cancellation for negative x) and its derivative
+// Phi(x) + x phi(x).
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * erfc_cheb(-0.70710678118654752f * x); }
 __device__ __forceinline__ float gelu_grad(float x) {
-  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+  return fmaf(x * 0.39894228040143268f, __expf(-0.5f * x * x), 0.5f * erfc_cheb(-0.70710678118654752f * x));
 }
 
 // v (the accumulated product) -> every output the epilogue asks for, except the transposed bf16 copy.
-__device__ __forceinline__ float finish(const Epi& e, int m, int n, float v) {
+// b: the bias of column n (read by the caller once per column, ahead of the stores).
+__device__ __forceinline__ float finish(const Epi& e, int m, int n, float v, float b) {
   v *= e.alpha;
   const long o = (long)m * e.ldc + n;
   if (e.accumulate) v += e.C[o];
-  if (e.bias) v += e.bias[n];
+  if (e.bias) v += b;
   if (e.Cpre) e.Cpre[o] = v;
   if (e.act == 1) v = gelu(v);
   if (e.C) e.C[o] = v;
@@ -189,6 +209,42 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
         }
     return;
   }
+  // each column's bias once, all loads in flight together: read after the first store, it would be read
+  // again for every element (a store may alias it), each time waiting out the memory latency
+  float bj[FN];
+#pragma unroll
+  for (int j = 0; j < FN; j++) {
+    const int n = n0 + wn + 16 * j + (lane & 15);
+    bj[j] = (g.e.bias && n < g.N) ? g.e.bias[n] : 0.f;
+  }
+  // a hidden layer's forward on a full tile (the bulk of every forward): bias, GELU, the bf16 activation
+  // and, in training, the pre-activation and the transposed bf16 copy, with no per-element bounds or flag
+  // tests and the row addresses formed once per fragment; the same values as the general form below
+  // (alpha 1: v * alpha is v exactly)
+  if (m0 + BM <= g.M && n0 + BN <= g.N && g.e.alpha == 1.f && !g.e.accumulate && g.e.bias && g.e.act == 1 &&
+      !g.e.C && g.e.Cbf) {
+    float* const pre = g.e.Cpre;
+    uint16_t* const cbT = g.e.CbfT;
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+      for (int j = 0; j < FN; j++) {
+        const int mb = m0 + wm + 16 * i + 4 * (lane >> 4), n = n0 + wn + 16 * j + (lane & 15);
+        uint16_t* cb = g.e.Cbf + (long)mb * g.e.ldcb + n;
+        uint16_t t[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const float v = acc[i][j][r] + bj[j];
+          if (pre) pre[(long)(mb + r) * g.e.ldc + n] = v;
+          t[r] = f2bf(gelu(v));
+          cb[(long)r * g.e.ldcb] = t[r];
+        }
+        if (cbT)  // four consecutive rows of column n: one 8-byte store
+          *reinterpret_cast<uint2*>(cbT + (long)n * g.e.ldct + mb) =
+              make_uint2(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16));
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < FM; i++)
 #pragma unroll
@@ -197,7 +253,8 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
       if (n >= g.N) continue;
       uint16_t t[4];
 #pragma unroll
-      for (int r = 0; r < 4; r++) t[r] = (mb + r < g.M) ? f2bf(finish(g.e, mb + r, n, acc[i][j][r])) : (uint16_t)0;
+      for (int r = 0; r < 4; r++)
+        t[r] = (mb + r < g.M) ? f2bf(finish(g.e, mb + r, n, acc[i][j][r], bj[j])) : (uint16_t)0;
       if (g.e.CbfT) {
         uint16_t* q = g.e.CbfT + (long)n * g.e.ldct + mb;
         if (mb + 4 <= g.M) {  // four consecutive rows of column n: one 8-byte store
@@ -216,7 +273,7 @@ __global__ void __launch_bounds__(256) gemm_reduce_kernel(GemmArgs g) {
   const int m = (int)(i / g.N), n = (int)(i % g.N);
   float v = 0.f;
   for (int s = 0; s < g.splits; s++) v += g.part[(long)s * g.M * g.N + i];
-  v = finish(g.e, m, n, v);
+  v = finish(g.e, m, n, v, g.e.bias ? g.e.bias[n] : 0.f);
   if (g.e.CbfT) g.e.CbfT[(long)n * g.e.ldct + m] = f2bf(v);
 }
 
@@ -511,7 +568,9 @@ int pack_weights(bmc_mlp* h, hipStream_t s) {
   return BMC_OK;
 }
 
-int forward(bmc_mlp* h, const float* X, int B, hipStream_t s) {
+// train: also keep what backward reads (the pre-activations and the transposed bf16 copies of the input
+// and of every hidden activation); inference writes only each layer's bf16 activations and the output.
+int forward(bmc_mlp* h, const float* X, int B, hipStream_t s, bool train) {
   TileArgs t{};
   t.mode = TILE_INPUT;
   t.rows = B;
@@ -522,7 +581,7 @@ int forward(bmc_mlp* h, const float* X, int B, hipStream_t s) {
   t.scale = h->scale;
   t.bf = h->x_bf;
   t.ldo = pad8(h->dims[0]);
-  t.bfT = h->x_bfT;
+  t.bfT = train ? h->x_bfT : nullptr;
   t.ldt = h->ldb;
   int rc = tiles(t, s);
   if (rc) return rc;
@@ -542,10 +601,10 @@ int forward(bmc_mlp* h, const float* X, int B, hipStream_t s) {
     g.e.act = last ? 0 : 1;
     g.e.ldc = N;
     g.e.C = last ? h->act[l] : nullptr;  // hidden layers need only the pre-activation and the bf16 copies
-    g.e.Cpre = last ? nullptr : h->pre[l];
+    g.e.Cpre = (last || !train) ? nullptr : h->pre[l];
     g.e.Cbf = last ? nullptr : h->act_bf[l];
     g.e.ldcb = pad8(N);
-    g.e.CbfT = last ? nullptr : h->act_bfT[l];
+    g.e.CbfT = (last || !train) ? nullptr : h->act_bfT[l];
     g.e.ldct = h->ldb;
     if ((rc = gemm(g, h->part, h->part_floats, s))) return rc;
   }
@@ -800,7 +859,7 @@ int bmc_mlp_forward(bmc_mlp* h, const float* x_dev, int batch, float* y_dev, voi
   }
   if (batch == 0) return BMC_OK;
   hipStream_t s = (hipStream_t)stream;
-  int rc = forward(h, x_dev, batch, s);
+  int rc = forward(h, x_dev, batch, s, false);
   if (rc) return rc;
   const int D = h->dims[h->L];
   BMC_HIP_CHECK(hipMemcpyAsync(y_dev, h->act[h->L - 1], 4L * batch * D, hipMemcpyDeviceToDevice, s));
@@ -815,7 +874,7 @@ int bmc_mlp_gradients(bmc_mlp* h, const float* x_dev, const float* t_dev, int ba
   }
   hipStream_t s = (hipStream_t)stream;
   BMC_HIP_CHECK(hipMemsetAsync(h->loss, 0, 4, s));
-  int rc = forward(h, x_dev, batch, s);
+  int rc = forward(h, x_dev, batch, s, true);
   if (!rc) rc = backward(h, t_dev, batch, s);
   if (rc) return rc;
   if (grad_host) BMC_HIP_CHECK(hipMemcpyAsync(grad_host, h->g, 4 * h->n_params, hipMemcpyDeviceToHost, s));
@@ -832,7 +891,7 @@ int bmc_mlp_train_step(bmc_mlp* h, const float* x_dev, const float* t_dev, int b
   }
   hipStream_t s = (hipStream_t)stream;
   float* lacc = loss_dev ? loss_dev : h->loss;
-  int rc = forward(h, x_dev, batch, s);
+  int rc = forward(h, x_dev, batch, s, true);
   if (!rc) {
     // the loss is accumulated into *lacc (the caller clears it)
     float* keep = h->loss;
