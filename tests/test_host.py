@@ -1,6 +1,6 @@
 """Host-side logic of the package that needs no GPU: the joint LM step with active bounds, the B0-spread
 quadrature, the host Levenberg-Marquardt's held entries, FitSpec validation (the reference pins these in
-tests/test_joint.py and tests/test_fitting.py)."""
+tests/test_joint.py and tests/test_fitting.py), and the MLP kernels' GELU fit against scipy."""
 import numpy as np
 import pytest
 
@@ -64,3 +64,36 @@ def test_free_param_validation():
         FreeParam("k", "cest_k", 0, 1.0, 0.0, 1.0, 0.5, scope="voxel")
     with pytest.raises(ValueError, match="m_start"):
         FreeParam("m", "m_start", None, 1.0, 0.0, 1.0, 0.5, mode="add")
+
+
+def _erfc_cheb(z):
+    """The MLP kernels' erfc (csrc/bmc_mlp.hip erfc_cheb), the Chebyshev fit of Numerical Recipes' erfcc,
+    evaluated in float32 as the kernels do."""
+    c = [0.17087277, -0.82215223, 1.48851587, -1.13520398, 0.27886807, -0.18628806, 0.09678418, 0.37409196,
+         1.00002368, -1.26551223]
+    z = np.asarray(z, np.float32)
+    a = np.abs(z)
+    t = np.float32(1) / (np.float32(1) + np.float32(0.5) * a)
+    p = np.float32(c[0])
+    for ck in c[1:]:
+        p = t * p + np.float32(ck)
+    e = t * np.exp(-a * a + p)
+    return np.where(z >= 0, e, np.float32(2) - e)
+
+
+def test_the_mlp_gelu_fit_against_scipy():
+    """GELU x erfc(-x / sqrt 2) / 2 with the fit's erfc against scipy's erfc: far below BF16's resolution
+    (the activations are rounded to BF16, 3.9e-3 relative), and a dropped coefficient is not."""
+    from scipy.special import erfc
+    x = np.linspace(-12.0, 12.0, 400001)
+    ref = 0.5 * x * erfc(-x / np.sqrt(2.0))
+    got = 0.5 * x * _erfc_cheb(-x / np.sqrt(2.0)).astype(np.float64)
+    assert np.max(np.abs(got - ref)) < 3e-7
+    big = np.abs(ref) > 1e-6
+    assert np.max(np.abs(got - ref)[big] / np.abs(ref[big])) < 5e-6
+    z = np.linspace(0.0, 6.0, 60001)
+    assert np.max(np.abs(_erfc_cheb(z) - erfc(z)) / erfc(z)) < 1e-5
+    # the plausible mistake of dropping the last coefficient of P
+    c0 = _erfc_cheb(z)
+    t = 1.0 / (1.0 + 0.5 * z)
+    assert np.max(np.abs(c0 / np.exp(0.17087277 * t ** 9) - erfc(z)) / erfc(z)) > 1e-2
