@@ -354,6 +354,25 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, int tile_r, int til
 
 __global__ void __launch_bounds__(256) tile_kernel(TileArgs a) { tile_body(a, blockIdx.y, blockIdx.x); }
 
+// The standardised input alone (inference: no transposed copy): four consecutive features per thread, a
+// 16-byte load and an 8-byte store (cols, ldx and ldo multiples of 4, x 16-byte aligned; the host checks).
+__global__ void __launch_bounds__(256) input_kernel(const float* x, long ldx, const float* shift, const float* scale,
+                                                    int rows, int cols, uint16_t* bf, long ldo) {
+  const int q = cols / 4;
+  const long n = (long)rows * q;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const int r = (int)(e / q), c = (int)(e % q) * 4;
+    const float4 v = *reinterpret_cast<const float4*>(x + (long)r * ldx + c);
+    float w[4] = {v.x, v.y, v.z, v.w};
+    if (shift) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) w[k] = (w[k] - shift[c + k]) * scale[c + k];
+    }
+    *reinterpret_cast<uint2*>(bf + (long)r * ldo + c) =
+        make_uint2(f2bf(w[0]) | ((uint32_t)f2bf(w[1]) << 16), f2bf(w[2]) | ((uint32_t)f2bf(w[3]) << 16));
+  }
+}
+
 // Every layer's weight copies (bf16 W and W^T from the fp32 master) in one launch: block b takes tile
 // b - first[l] of the layer l whose range holds it.
 constexpr int kMaxPackLayers = 8;
@@ -583,7 +602,16 @@ int forward(bmc_mlp* h, const float* X, int B, hipStream_t s, bool train) {
   t.ldo = pad8(h->dims[0]);
   t.bfT = train ? h->x_bfT : nullptr;
   t.ldt = h->ldb;
-  int rc = tiles(t, s);
+  int rc = BMC_OK;
+  if (!train && B > 0 && t.cols % 4 == 0 && t.ldx % 4 == 0 && t.ldo % 4 == 0 &&
+      reinterpret_cast<uintptr_t>(X) % 16 == 0) {
+    const long n = (long)B * (t.cols / 4);
+    const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(input_kernel, dim3(blocks), dim3(256), 0, s, X, t.ldx, t.aux, t.scale, B, t.cols, t.bf, t.ldo);
+    BMC_HIP_CHECK(hipGetLastError());
+  } else {
+    rc = tiles(t, s);
+  }
   if (rc) return rc;
   for (int l = 0; l < h->L; l++) {
     const int K = h->dims[l], N = h->dims[l + 1];
