@@ -14,8 +14,9 @@
 // elements so every 16-byte load is aligned; reads past the logical K are masked to zero.
 //
 // The kernel: 256 threads (2 x 2 waves) per BM x BN tile (128 x 128 or 64 x 64), K in slabs of 64
-// staged through double-buffered LDS, v_mfma_f32_16x16x32_bf16 (lane l: A row l % 16 with k = 8 (l / 16)
-// .. + 7, B column l % 16 with the same k, D column l % 16 with rows 4 (l / 16) .. + 3). Products with too
+// staged through double-buffered LDS, v_mfma_f32_32x32x16_bf16 (lane l: A row l % 32 with k = 8 (l / 32)
+// .. + 7, B column l % 32 with the same k, D column l % 32 with rows 8 q + 4 (l / 32) .. + 3 in register
+// group q; per flop half the LDS reads of the 16 x 16 x 32 form, which bounded the loop). Products with too
 // few tiles to fill 256 CUs (the weight gradients, K = batch) split K over the grid's z axis into FP32
 // partials that a second kernel adds in a fixed order (training stays bitwise reproducible).
 #include <math.h>
@@ -32,7 +33,7 @@ namespace bmc {
 namespace mlp {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   // round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
@@ -124,22 +125,42 @@ __device__ __forceinline__ uint4 load8(const uint16_t* base, long ld, int row, i
                     t[6] | ((uint32_t)t[7] << 16));
 }
 
+// The first of the four consecutive rows that register group q (registers 4 q .. 4 q + 3) of a 32 x 32
+// v_mfma_f32_32x32x16_bf16 result holds in lane `lane`, for a fragment starting at row r0: the lane holds
+// column lane % 32 at rows 8 q + 4 (lane / 32) + (0..3).
+__device__ __forceinline__ int frag_row(int r0, int lane, int q) {
+  return r0 + 8 * q + 4 * (lane >> 5);
+}
+
 template <int BM, int BN>
 __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
-  constexpr int FM = BM / 32, FN = BN / 32;        // 16 x 16 fragments per wave (wave tile BM/2 x BN/2)
+  constexpr int FM = BM / 64, FN = BN / 64;        // 32 x 32 fragments per wave (wave tile BM/2 x BN/2)
   constexpr int CA = BM * BK / 8 / 256, CB = BN * BK / 8 / 256;  // 16-byte chunks per thread per slab
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][BM * LDS_ROW];
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][BN * LDS_ROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by linear id (x fastest), and each XCD
+  // has its own L2. In grid order the gridDim.x column tiles of one row block would run on different
+  // XCDs, each fetching that row block of A from HBM; remapped, XCD x takes row blocks x, x + 8, ... with
+  // all their column tiles in a row, so a row block of A is fetched once. A bijection when gridDim.y is
+  // a multiple of 8 (else, and for split-K grids, the grid order is kept).
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (gridDim.z == 1 && (gridDim.y & 7) == 0) {
+    const int b = blockIdx.y * gridDim.x + blockIdx.x, k = b >> 3;
+    by = (b & 7) + 8 * (k / (int)gridDim.x);
+    bx = k % (int)gridDim.x;
+  }
+  const int m0 = by * BM, n0 = bx * BN;
   const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
   const int kbeg = blockIdx.z * g.kchunk;
   const int kend = min(g.K, kbeg + g.kchunk);
-  floatx4 acc[FM][FN];
+  floatx16 acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; i++)
 #pragma unroll
-    for (int j = 0; j < FN; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < FN; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
 
   uint4 ra[CA], rb[CB];
   auto fetch = [&](int k0) {
@@ -177,19 +198,19 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
     const bool more = k0 + BK < kend;
     if (more) fetch(k0 + BK);  // the next slab's global loads overlap this slab's MFMAs
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 32) {
-      const int kk = ks + 8 * (lane >> 4);
+    for (int ks = 0; ks < BK; ks += 16) {
+      const int kk = ks + 8 * (lane >> 5);
       bf16x8 a[FM], b[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++)
-        a[i] = *reinterpret_cast<const bf16x8*>(&sA[buf][(wm + 16 * i + (lane & 15)) * LDS_ROW + kk]);
+        a[i] = *reinterpret_cast<const bf16x8*>(&sA[buf][(wm + 32 * i + (lane & 31)) * LDS_ROW + kk]);
 #pragma unroll
       for (int j = 0; j < FN; j++)
-        b[j] = *reinterpret_cast<const bf16x8*>(&sB[buf][(wn + 16 * j + (lane & 15)) * LDS_ROW + kk]);
+        b[j] = *reinterpret_cast<const bf16x8*>(&sB[buf][(wn + 32 * j + (lane & 31)) * LDS_ROW + kk]);
 #pragma unroll
       for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (more) stash(buf ^ 1);
     __syncthreads();
@@ -203,8 +224,8 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
 #pragma unroll
       for (int j = 0; j < FN; j++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r, n = n0 + wn + 16 * j + (lane & 15);
+        for (int r = 0; r < 16; r++) {
+          const int m = frag_row(m0 + wm + 32 * i, lane, r >> 2) + (r & 3), n = n0 + wn + 32 * j + (lane & 31);
           if (m < g.M && n < g.N) P[(long)m * g.N + n] = acc[i][j][r];
         }
     return;
@@ -214,7 +235,7 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
   float bj[FN];
 #pragma unroll
   for (int j = 0; j < FN; j++) {
-    const int n = n0 + wn + 16 * j + (lane & 15);
+    const int n = n0 + wn + 32 * j + (lane & 31);
     bj[j] = (g.e.bias && n < g.N) ? g.e.bias[n] : 0.f;
   }
   // a hidden layer's forward on a full tile (the bulk of every forward): bias, GELU, the bf16 activation
@@ -228,13 +249,15 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
 #pragma unroll
     for (int i = 0; i < FM; i++)
 #pragma unroll
-      for (int j = 0; j < FN; j++) {
-        const int mb = m0 + wm + 16 * i + 4 * (lane >> 4), n = n0 + wn + 16 * j + (lane & 15);
+      for (int j = 0; j < FN; j++)
+#pragma unroll
+      for (int q4 = 0; q4 < 4; q4++) {
+        const int mb = frag_row(m0 + wm + 32 * i, lane, q4), n = n0 + wn + 32 * j + (lane & 31);
         uint16_t* cb = g.e.Cbf + (long)mb * g.e.ldcb + n;
         uint16_t t[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const float v = acc[i][j][r] + bj[j];
+          const float v = acc[i][j][4 * q4 + r] + bj[j];
           if (pre) pre[(long)(mb + r) * g.e.ldc + n] = v;
           t[r] = f2bf(gelu(v));
           cb[(long)r * g.e.ldcb] = t[r];
@@ -248,13 +271,15 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
 #pragma unroll
   for (int i = 0; i < FM; i++)
 #pragma unroll
-    for (int j = 0; j < FN; j++) {
-      const int mb = m0 + wm + 16 * i + 4 * (lane >> 4), n = n0 + wn + 16 * j + (lane & 15);
+    for (int j = 0; j < FN; j++)
+#pragma unroll
+    for (int q4 = 0; q4 < 4; q4++) {
+      const int mb = frag_row(m0 + wm + 32 * i, lane, q4), n = n0 + wn + 32 * j + (lane & 31);
       if (n >= g.N) continue;
       uint16_t t[4];
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        t[r] = (mb + r < g.M) ? f2bf(finish(g.e, mb + r, n, acc[i][j][r], bj[j])) : (uint16_t)0;
+        t[r] = (mb + r < g.M) ? f2bf(finish(g.e, mb + r, n, acc[i][j][4 * q4 + r], bj[j])) : (uint16_t)0;
       if (g.e.CbfT) {
         uint16_t* q = g.e.CbfT + (long)n * g.e.ldct + mb;
         if (mb + 4 <= g.M) {  // four consecutive rows of column n: one 8-byte store
