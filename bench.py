@@ -775,7 +775,7 @@ def nn_record(args, log=None):
                                "MLP 244-512-512-256-5 GELU, input standardised", "n_train": args.nn_train,
                    "epochs": args.nn_epochs, "n_test": args.nn_test},
         "train_seconds": r["train_seconds"], "final_train_loss": r["final_train_loss"],
-        "roofline": {"bound": "mfma", "kernel": "gemm_nt_kernel (bf16 MFMA 16x16x32, fp32 accumulate)",
+        "roofline": {"bound": "mfma", "kernel": "gemm_nt_kernel (bf16 MFMA 32x32x16, fp32 accumulate)",
                      "achieved": inf_tflops, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": inf_tflops / PEAK_BF16_TFLOPS,
                      "measured_on": f"MLP inference over {Zt.shape[0]} spectra (batches of {prop.mlp.max_batch}), "
