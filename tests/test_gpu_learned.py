@@ -138,3 +138,23 @@ def test_an_inference_copy_gives_the_same_outputs(torch_mod):
     scale_y = max(1.0, np.abs(Yo).max())
     assert e_big.max() <= max(1.5 * e_small.max(), 2e-3 * scale_y)
     assert np.median(e_big) < 1e-4 * scale_y
+
+
+@pytest.mark.parametrize("batch", [8192, 8200])
+def test_large_batch_forward_matches_the_oracle(torch_mod, batch):
+    """Batches whose 512-wide layers have at least 256 tiles of 128 x 128 run the large-tile GEMM (32 x 32
+    fragments; at 8,192 rows, 64 row blocks, in the XCD-aware tile order; at 8,200 rows, 65 row blocks
+    with a ragged last one, in grid order) and the full-tile epilogue: the outputs against the oracle."""
+    torch = torch_mod
+    from bmc_jax.learned import DIMS, MLP
+    mlp = MLP(dims=DIMS, max_batch=8200, seed=13)
+    rng = np.random.default_rng(4)
+    X = (0.9 - 0.3 * np.sin(np.linspace(0, 3, DIMS[0])) + 0.03 * rng.normal(size=(batch, DIMS[0]))).astype(np.float32)
+    Xd = torch.as_tensor(X, device="cuda")
+    mlp.fit_input_norm(Xd)
+    shift, scale = mlp.input_norm()
+    Y = mlp.forward(Xd).cpu().numpy()
+    Yo, _, _ = MO.forward(mlp.get_params().astype(np.float64), DIMS, X, shift, scale)
+    scale_y = max(1.0, np.abs(Yo).max())
+    assert np.max(np.abs(Y - Yo)) < 2e-3 * scale_y
+    assert np.median(np.abs(Y - Yo)) < 1e-4 * scale_y
