@@ -790,6 +790,41 @@ def nn_record(args, log=None):
     return out
 
 
+def bench_mlp(args):
+    """The learned proposal's network alone: BF16 MFMA inference of 244-512-512-256-5 over 65,536
+    synthetic Z-spectrum-like inputs per launch (random weights, inputs standardised), timed with HIP
+    events on the launch stream over `steps` forwards."""
+    import torch
+    from bmc_jax import learned
+    n = 65536
+    mlp = learned.MLP(dims=learned.DIMS, max_batch=n, seed=1)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    profile = 0.9 - 0.3 * torch.sin(torch.linspace(0, 3, learned.DIMS[0], device="cuda"))
+    X = profile + 0.03 * torch.randn(n, learned.DIMS[0], device="cuda", generator=g)
+    mlp.fit_input_norm(X)
+    for _ in range(max(args.warmup, 1)):
+        mlp.forward(X)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        mlp.forward(X)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t = 1e-3 * e0.elapsed_time(e1) / args.steps
+    dims = learned.DIMS
+    flops = 2.0 * n * sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
+    print(json.dumps({"metric": "MLP inference spectra/s (244-512-512-256-5, BF16 MFMA)", "value": n / t,
+                      "unit": "spectra/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": 1e3 * t, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "bf16 / fp32 accumulate", "data": "synthetic (random weights and inputs)",
+                      "config": {"workload": "MLP inference, 65,536 spectra per launch"},
+                      "roofline": {"bound": "mfma", "achieved": flops / t / 1e12, "peak": 2500.0,
+                                   "unit": "TFLOP/s", "frac": flops / t / 1e12 / 2500.0, "traffic": None}}),
+          flush=True)
+
+
 def bench_squaring(args):
     """The squarings' dense product on one GPU, two ways (bmc_jax._probe): the library's team column product
     (mul_cols: DPP or ds_bpermute broadcasts of the left factor's columns) against one FP64 MFMA tile
@@ -866,7 +901,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 1 GiB device buffer (the reference point for scaling HBM counters)")
     ap.add_argument("--workload", choices=("pulsed", "nn", "design", "uneven", "scanner", "joint", "tutorial08",
-                                           "tutorial09", "tutorial10", "tutorial11", "squaring"), default="pulsed",
+                                           "tutorial09", "tutorial10", "tutorial11", "squaring", "mlp"), default="pulsed",
                     help="pulsed: forward + Jacobian Z-spectra/s (default); nn: fitted voxels/s, NN-started vs plain LM")
     ap.add_argument("--nn-train", type=int, default=2_000_000)
     ap.add_argument("--nn-test", type=int, default=262_144)
@@ -900,6 +935,10 @@ def main():
     if args.workload == "squaring":
         _native.torch_cuda()
         bench_squaring(args)
+        return
+    if args.workload == "mlp":
+        _native.torch_cuda()
+        bench_mlp(args)
         return
     if args.workload == "scanner":
         _native.torch_cuda()
