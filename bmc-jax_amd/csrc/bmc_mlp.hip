@@ -163,7 +163,24 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
       for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
 
   uint4 ra[CA], rb[CB];
+  // a slab wholly inside both operands (the tile's rows and the slab's K; a uniform test) is read with
+  // plain 16-byte loads, all in flight together: load8's element-wise edge path, joined after every load,
+  // made the compiler wait for each load before issuing the next
+  const bool rows_in = m0 + BM <= g.M && n0 + BN <= g.N;
   auto fetch = [&](int k0) {
+    if (rows_in && k0 + BK <= kend) {
+#pragma unroll
+      for (int c = 0; c < CA; c++) {
+        const int e = tid + c * 256, row = e >> 3, kc = (e & 7) * 8;
+        ra[c] = *reinterpret_cast<const uint4*>(g.A + (long)(m0 + row) * g.lda + k0 + kc);
+      }
+#pragma unroll
+      for (int c = 0; c < CB; c++) {
+        const int e = tid + c * 256, row = e >> 3, kc = (e & 7) * 8;
+        rb[c] = *reinterpret_cast<const uint4*>(g.B + (long)(n0 + row) * g.ldb + k0 + kc);
+      }
+      return;
+    }
 #pragma unroll
     for (int c = 0; c < CA; c++) {
       const int e = tid + c * 256, row = e >> 3, kc = (e & 7) * 8;
